@@ -1,0 +1,177 @@
+/*
+ * pyronn_mi.h - C ABI of the MI355X (gfx950) projector / backprojector library
+ * libpyronn_mi.so.
+ *
+ * One entry point per reference operator.  Each replaces the reference's TF
+ * op Compute() plus its CUDA launcher for that operator; the citation above
+ * each declaration names the REGISTER_OP input list and the launcher it
+ * replaces (paths relative to the reference repository root).
+ *
+ * Conventions shared by every entry point
+ *  - All array arguments are DEVICE pointers, batch-major, densely packed,
+ *    exactly as the reference op receives its tensors:
+ *      volume_origin / volume_spacing   float[B][3] (z,y,x) for 3D, float[B][2] (y,x) for 2D
+ *      projection_matrices              float[B][n][3][4] row-major
+ *      ray_vectors / central_ray_vectors float[B][n][2] (x,y)
+ *      step_size, projection_multiplier, detector_origin, detector_spacing,
+ *      source_2_isocenter_distance, source_2_detector_distance    float[B][1]
+ *    Scalars stay on the device (the reference dereferences them in-kernel);
+ *    shapes are host integers (the reference copies its shape tensor to the
+ *    host).  Outputs are always float32 and are overwritten, never accumulated.
+ *  - `*_dtype` selects the storage type of the data input: PYRONN_F32,
+ *    PYRONN_BF16 or PYRONN_F16.  Arithmetic is float32 in every case.
+ *  - Calls are asynchronous on `stream` (NULL = the null stream), allocate no
+ *    device memory and keep no state between calls: the caller passes the
+ *    workspace sized by the matching *_workspace_size() function.  Calls are
+ *    reentrant; two calls may run concurrently if their workspaces differ.
+ *  - Return value: PYRONN_OK or an error code; pyronn_last_error() gives the
+ *    message for the calling thread.  Invalid arguments are rejected before
+ *    anything is launched.  Invalid *values* that only the device sees (a
+ *    singular projection matrix, a step size <= 0 or NaN) produce NaN in the
+ *    affected output elements instead of a hang or a process exit.
+ */
+#ifndef PYRONN_MI_H
+#define PYRONN_MI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *pyronn_stream_t; /* == hipStream_t */
+
+enum pyronn_status {
+  PYRONN_OK = 0,
+  PYRONN_ERR_INVALID_ARGUMENT = 1,
+  PYRONN_ERR_WORKSPACE = 2,
+  PYRONN_ERR_TOO_LARGE = 3,
+  PYRONN_ERR_HIP = 4
+};
+
+enum pyronn_dtype { PYRONN_F32 = 0, PYRONN_BF16 = 1, PYRONN_F16 = 2 };
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *pyronn_last_error(void);
+
+/* Library version as 10000*major + 100*minor + patch. */
+int pyronn_version(void);
+
+/*
+ * Per-view cone geometry on the device, in FP64, from the projection matrices:
+ * inv_ar[b][v] = diag(1/spacing_xyz) * M^-1 (row-major 3x3) and
+ * src[b][v] = (C - origin_xyz) / spacing_xyz with C = -M^-1 p4 the camera
+ * centre.  Singular M gives NaN rows.
+ * Replaces ConeProjection3DOp::DecomposeProjectionMatrices
+ * (cc/ops/cone_projector_3D_OPKernel.cc:96-182) and Geometry::getCameraCenter
+ * (cc/helper_headers/helper_geometry_cpu.h:30-43), which ran on the host with
+ * Eigen's JacobiSVD after synchronous device-to-host copies.
+ */
+int pyronn_cone_geometry_setup(const float *projection_matrices, const float *volume_origin,
+                               const float *volume_spacing, int batch, int n_views, float *inv_ar,
+                               float *src, pyronn_stream_t stream);
+
+/* Workspace bytes for pyronn_cone_projection3d with these arguments. */
+size_t pyronn_cone_projection3d_workspace_size(int batch, int D, int H, int W, int n_views,
+                                               int volume_dtype);
+
+/*
+ * ConeProjection3D: volume [B][D][H][W] -> sinogram [B][n][H_d][W_d] (float32).
+ * Ray-driven trilinear line integrals, step `step_size` in voxel units,
+ * hardware_interp selects the texture-mode clip box [-0.5, N-0.5] (exact
+ * weights) instead of [0, N].  projection_multiplier is accepted and ignored,
+ * as in the reference.
+ * Replaces REGISTER_OP("ConeProjection3D") (cc/ops/cone_projector_3D_OPKernel.cc:34-60),
+ * ConeProjection3DOp::Compute (:185-248), Cone_Projection_Kernel_Launcher
+ * (cc/kernels/cone_projector_3D_CudaKernel.cu.cc:225-257) and
+ * Cone_Projection_Kernel_Tex_Interp_Launcher
+ * (cc/kernels/cone_projector_3D_CudaKernel_hardware_interp.cu.cc:199-262).
+ */
+int pyronn_cone_projection3d(const void *volume, int volume_dtype, int batch, int D, int H, int W,
+                             const float *volume_origin, const float *volume_spacing,
+                             const float *projection_matrices, const float *step_size,
+                             const float *projection_multiplier, int n_views, int det_h, int det_w,
+                             int hardware_interp, float *sinogram, void *workspace,
+                             size_t workspace_bytes, pyronn_stream_t stream);
+
+/* Workspace bytes for pyronn_cone_backprojection3d with these arguments. */
+size_t pyronn_cone_backprojection3d_workspace_size(int batch, int n_views, int det_h, int det_w,
+                                                   int sinogram_dtype);
+
+/*
+ * ConeBackprojection3D: sinogram [B][n][H_d][W_d] -> volume [B][D][H][W] (float32).
+ * Voxel-driven: vol = projection_multiplier * sum_n bilinear(sino_n, P_n x) / w_n^2,
+ * zero outside the detector (the reference's off-by-one bounds test, Q11, is
+ * not reproduced).  step_size is accepted and ignored, as in the reference.
+ * Replaces REGISTER_OP("ConeBackprojection3D") (cc/ops/cone_backprojector_3D_OPKernel.cc:27-53),
+ * ConeBackprojection3DOp::Compute (:75-140), Cone_Backprojection3D_Kernel_Launcher
+ * (cc/kernels/cone_backprojector_3D_CudaKernel.cu.cc:121-146) and
+ * Cone_Backprojection3D_Kernel_Tex_Interp_Launcher
+ * (cc/kernels/cone_backprojector_3D_CudaKernel_hardware_interp.cu.cc:104-160).
+ */
+int pyronn_cone_backprojection3d(const void *sinogram, int sinogram_dtype, int batch, int n_views,
+                                 int det_h, int det_w, const float *volume_origin,
+                                 const float *volume_spacing, const float *projection_matrices,
+                                 const float *step_size, const float *projection_multiplier, int D,
+                                 int H, int W, int hardware_interp, float *volume, void *workspace,
+                                 size_t workspace_bytes, pyronn_stream_t stream);
+
+/*
+ * ParallelProjection2D: volume [B][H][W] -> sinogram [B][n][det].
+ * step_factor scales min(spacing) to the march step; the reference hard-codes
+ * 0.2 (cc/kernels/par_projector_2D_CudaKernel.cu.cc:166), pass 0.2 for parity.
+ * Replaces REGISTER_OP("ParallelProjection2D") (cc/ops/par_projector_2D_OPKernel.cc:30-56),
+ * ParallelProjection2DOp::Compute (:67-120) and Parallel_Projection2D_Kernel_Launcher
+ * (cc/kernels/par_projector_2D_CudaKernel.cu.cc:149-178).
+ */
+int pyronn_parallel_projection2d(const void *volume, int volume_dtype, int batch, int H, int W,
+                                 const float *volume_origin, const float *detector_origin,
+                                 const float *volume_spacing, const float *detector_spacing,
+                                 const float *ray_vectors, int n_views, int det,
+                                 float step_factor, float *sinogram, pyronn_stream_t stream);
+
+/*
+ * ParallelBackprojection2D: sinogram [B][n][det] -> volume [B][H][W].
+ * Replaces REGISTER_OP("ParallelBackprojection2D") (cc/ops/par_backprojector_2D_OPKernel.cc:26-51),
+ * its Compute (:63-115) and Parallel_Backprojection2D_Kernel_Launcher
+ * (cc/kernels/par_backprojector_2D_CudaKernel.cu.cc:77-105).
+ */
+int pyronn_parallel_backprojection2d(const void *sinogram, int sinogram_dtype, int batch,
+                                     int n_views, int det, const float *volume_origin,
+                                     const float *detector_origin, const float *volume_spacing,
+                                     const float *detector_spacing, const float *ray_vectors,
+                                     int H, int W, float *volume, pyronn_stream_t stream);
+
+/*
+ * FanProjection2D: volume [B][H][W] -> sinogram [B][n][det].
+ * Replaces REGISTER_OP("FanProjection2D") (cc/ops/fan_projector_2D_OPKernel.cc:26-53),
+ * FanProjection2DOp::Compute (:66-128) and Fan_Projection_Kernel_Launcher
+ * (cc/kernels/fan_projector_2D_CudaKernel.cu.cc:165-194).
+ */
+int pyronn_fan_projection2d(const void *volume, int volume_dtype, int batch, int H, int W,
+                            const float *volume_origin, const float *detector_origin,
+                            const float *volume_spacing, const float *detector_spacing,
+                            const float *source_2_isocenter_distance,
+                            const float *source_2_detector_distance,
+                            const float *central_ray_vectors, int n_views, int det,
+                            float step_factor, float *sinogram, pyronn_stream_t stream);
+
+/*
+ * FanBackprojection2D: sinogram [B][n][det] -> volume [B][H][W].
+ * Replaces REGISTER_OP("FanBackprojection2D") (cc/ops/fan_backprojector_2D_OPKernel.cc:26-53),
+ * its Compute (:67-128) and Fan_Backprojection2D_Kernel_Launcher
+ * (cc/kernels/fan_backprojector_2D_CudaKernel.cu.cc:83-113).
+ */
+int pyronn_fan_backprojection2d(const void *sinogram, int sinogram_dtype, int batch, int n_views,
+                                int det, const float *volume_origin, const float *detector_origin,
+                                const float *volume_spacing, const float *detector_spacing,
+                                const float *source_2_isocenter_distance,
+                                const float *source_2_detector_distance,
+                                const float *central_ray_vectors, int H, int W, float *volume,
+                                pyronn_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYRONN_MI_H */

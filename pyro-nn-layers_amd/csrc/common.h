@@ -1,0 +1,90 @@
+// Shared device helpers and host-side error plumbing for libpyronn_mi.so.
+//
+// Arithmetic conventions (identical in every kernel of the library):
+//  * The library is compiled with -ffp-contract=off; every fused multiply-add
+//    is written out with fmaf().  Where the reference writes `a*b + c`, the
+//    kernels use fmaf(a, b, c), the contraction nvcc applies by default.
+//  * Division and square root are IEEE correctly rounded
+//    (-fhip-fp32-correctly-rounded-divide-sqrt); rsqrtf(x) of the reference is
+//    evaluated as 1.0f / sqrtf(x).
+//  * Samplers clamp the coordinate into [-1, N] before the floor and read from
+//    a copy of the data surrounded by a zero apron (1 element before, 2 after
+//    each interpolated axis).  For every finite coordinate this is exactly the
+//    reference's "zero outside [0, N-1]" rule without a bounds test per
+//    neighbour; NaN coordinates read zeros.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "pyronn_mi.h"
+
+namespace pyronn {
+
+// ----------------------------------------------------------------- storage types
+struct bf16s {
+  uint16_t bits;
+};
+
+__device__ __forceinline__ float to_f32(float x) { return x; }
+__device__ __forceinline__ float to_f32(bf16s x) { return __uint_as_float((uint32_t)x.bits << 16); }
+__device__ __forceinline__ float to_f32(_Float16 x) { return (float)x; }
+
+template <typename TS> __device__ __forceinline__ TS zero_of();
+template <> __device__ __forceinline__ float zero_of<float>() { return 0.0f; }
+template <> __device__ __forceinline__ bf16s zero_of<bf16s>() { return bf16s{0}; }
+template <> __device__ __forceinline__ _Float16 zero_of<_Float16>() { return (_Float16)0.0f; }
+
+inline size_t dtype_bytes(int dt) { return dt == PYRONN_F32 ? 4 : 2; }
+inline bool dtype_ok(int dt) { return dt == PYRONN_F32 || dt == PYRONN_BF16 || dt == PYRONN_F16; }
+
+// ----------------------------------------------------------------- arithmetic
+// Linear blend a + t (b - a) written as the reference writes it:
+// __fmaf_rn(b, t, __fmaf_rn(a, -t, a)) (cone_projector_3D_CudaKernel.cu.cc:74-82).
+__device__ __forceinline__ float lerp_ref(float a, float b, float t) {
+  return fmaf(b, t, fmaf(a, -t, a));
+}
+
+__device__ __forceinline__ float clamp_apron(float c, float n) {
+  return fminf(fmaxf(c, -1.0f), n);
+}
+
+// Iteration cap of every ray march (the reference has none and would spin
+// forever when the step is below the float resolution of alpha).
+__device__ __forceinline__ int march_cap(float amin, float amax, float step) {
+  if (!(amax > amin)) return 0;
+  const float n = ceilf(1.25f * (amax - amin) / step) + 16.0f;
+  return n < 4194304.0f ? (int)n : 4194304;
+}
+
+// Sizes of the zero-apron copies.
+constexpr int kApronLo = 1;
+constexpr int kApronExtra = 3;  // 1 before + 2 after
+
+// ----------------------------------------------------------------- host errors
+void set_error(const char *fmt, ...);
+void clear_error();
+int hip_fail(hipError_t e, const char *what);
+
+#define PYRONN_CHECK_ARG(cond, ...)                        \
+  do {                                                    \
+    if (!(cond)) {                                        \
+      ::pyronn::set_error(__VA_ARGS__);                   \
+      return PYRONN_ERR_INVALID_ARGUMENT;                 \
+    }                                                     \
+  } while (0)
+
+#define PYRONN_CHECK_LAUNCH(what)                                           \
+  do {                                                                     \
+    hipError_t e_ = hipGetLastError();                                     \
+    if (e_ != hipSuccess) return ::pyronn::hip_fail(e_, what);             \
+  } while (0)
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace pyronn

@@ -1,0 +1,494 @@
+// Cone-beam projector / backprojector for gfx950 (CDNA4).
+//
+// Kernels
+//   cone_setup_kernel   per-view FP64 geometry (M^-1, camera centre) on the device
+//   apron_copy_kernel   zero-apron copy (and storage-type passthrough) of the
+//                       volume / sinogram that the samplers read
+//   cone_fwd_kernel     ray-driven forward projection, one lane per detector pixel
+//   cone_bp_kernel      voxel-driven backprojection, one lane per ZPT voxels in z
+//
+// The forward march and the backprojection sum follow the reference kernels
+// operation by operation (see the citations at each kernel), so that results
+// agree with the host restatement to the last bit in almost every element.
+#include "common.h"
+
+namespace pyronn {
+
+// =====================================================================================
+// Per-view geometry (replaces cone_projector_3D_OPKernel.cc:96-182 and
+// helper_geometry_cpu.h:30-43).  FP64 closed form: C = -M^-1 p4 is the right null
+// vector of P with C(3) = 1; M^-1 = adj(M) / det(M).
+// =====================================================================================
+__global__ void cone_setup_kernel(const float *__restrict__ P, const float *__restrict__ origin,
+                                  const float *__restrict__ spacing, int batch, int n,
+                                  float *__restrict__ inv_ar, float *__restrict__ src) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= batch * n) return;
+  const int b = idx / n;
+  double m[12];
+  for (int i = 0; i < 12; ++i) m[i] = (double)P[(size_t)idx * 12 + i];
+  const double a = m[0], bb = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9],
+               k = m[10];
+  const double A = e * k - f * h, B = f * g - d * k, C = d * h - e * g;
+  const double det = a * A + bb * B + c * C;
+  double inv[9], s[3];
+  bool ok = (fabs(det) > 1e-300) && isfinite(det);
+  if (ok) {
+    const double adj[9] = {A, c * h - bb * k, bb * f - c * e, B, a * k - c * g,
+                           c * d - a * f, C, bb * g - a * h, a * e - bb * d};
+    double mi[9];
+    for (int i = 0; i < 9; ++i) mi[i] = adj[i] / det;
+    const double p4[3] = {m[3], m[7], m[11]};
+    const double sp[3] = {(double)spacing[b * 3 + 2], (double)spacing[b * 3 + 1],
+                          (double)spacing[b * 3 + 0]};
+    const double og[3] = {(double)origin[b * 3 + 2], (double)origin[b * 3 + 1],
+                          (double)origin[b * 3 + 0]};
+    for (int r = 0; r < 3; ++r) {
+      const double cr = -(mi[r * 3 + 0] * p4[0] + mi[r * 3 + 1] * p4[1] + mi[r * 3 + 2] * p4[2]);
+      s[r] = (cr - og[r]) / sp[r];
+      for (int cc = 0; cc < 3; ++cc) inv[r * 3 + cc] = mi[r * 3 + cc] / sp[r];
+    }
+    for (int i = 0; i < 9; ++i) ok = ok && isfinite(inv[i]);
+    for (int i = 0; i < 3; ++i) ok = ok && isfinite(s[i]);
+  }
+  const float qnan = __builtin_nanf("");
+  for (int i = 0; i < 9; ++i) inv_ar[(size_t)idx * 9 + i] = ok ? (float)inv[i] : qnan;
+  for (int i = 0; i < 3; ++i) src[(size_t)idx * 3 + i] = ok ? (float)s[i] : qnan;
+}
+
+// =====================================================================================
+// Zero-apron copy: src [outer][Z][H][W] -> dst [outer][Z + zx][H + 3][W + 3], data at
+// offset (zlo, 1, 1).  For the volume zx = 3, zlo = 1 (z is interpolated); for the
+// sinogram zx = 0 (views are not).
+// =====================================================================================
+template <typename TS>
+__global__ void apron_copy_kernel(const TS *__restrict__ src, TS *__restrict__ dst, int Z, int H,
+                                  int W, int zx, int zlo, long long total) {
+  const int Wp = W + kApronExtra, Hp = H + kApronExtra, Zp = Z + zx;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    long long r = t;
+    const int xp = (int)(r % Wp); r /= Wp;
+    const int yp = (int)(r % Hp); r /= Hp;
+    const int zp = (int)(r % Zp);
+    const long long o = r / Zp;
+    const int x = xp - kApronLo, y = yp - kApronLo, z = zp - zlo;
+    TS v = zero_of<TS>();
+    if (x >= 0 && x < W && y >= 0 && y < H && z >= 0 && z < Z)
+      v = src[((o * Z + z) * (long long)H + y) * W + x];
+    dst[t] = v;
+  }
+}
+
+// =====================================================================================
+// Trilinear sample from the zero-apron volume, lerp z -> y -> x
+// (cone_projector_3D_CudaKernel.cu.cc:42-85).
+// =====================================================================================
+template <typename TS>
+__device__ __forceinline__ float sample3(const TS *__restrict__ v, int Wp, int HWp, float px,
+                                         float py, float pz, float fW, float fH, float fD) {
+  px = clamp_apron(px, fW);
+  py = clamp_apron(py, fH);
+  pz = clamp_apron(pz, fD);
+  const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+  const float xd = px - fx, yd = py - fy, zd = pz - fz;
+  const int base = ((int)fz + kApronLo) * HWp + ((int)fy + kApronLo) * Wp + ((int)fx + kApronLo);
+  const float p000 = to_f32(v[base]), p001 = to_f32(v[base + 1]);
+  const float p010 = to_f32(v[base + Wp]), p011 = to_f32(v[base + Wp + 1]);
+  const float p100 = to_f32(v[base + HWp]), p101 = to_f32(v[base + HWp + 1]);
+  const float p110 = to_f32(v[base + HWp + Wp]), p111 = to_f32(v[base + HWp + Wp + 1]);
+  const float p00 = lerp_ref(p000, p100, zd);
+  const float p01 = lerp_ref(p001, p101, zd);
+  const float p10 = lerp_ref(p010, p110, zd);
+  const float p11 = lerp_ref(p011, p111, zd);
+  const float p0 = lerp_ref(p00, p10, yd);
+  const float p1 = lerp_ref(p01, p11, yd);
+  return lerp_ref(p0, p1, xd);
+}
+
+template <typename TS, bool HW>
+__device__ __forceinline__ float cone_sample(const TS *__restrict__ v, int Wp, int HWp, float px,
+                                             float py, float pz, float fW, float fH, float fD) {
+  if (HW) {  // tex3D(p + 0.5): the sampler subtracts the half texel again
+    px = (px + 0.5f) - 0.5f;
+    py = (py + 0.5f) - 0.5f;
+    pz = (pz + 0.5f) - 0.5f;
+  }
+  return sample3(v, Wp, HWp, px, py, pz, fW, fH, fD);
+}
+
+// =====================================================================================
+// Forward projection.  One lane per (view, row, col); a 256-lane block covers a
+// 16 x 16 detector tile and each wave an 8 x 8 sub-tile, so the 64 rays of a wave
+// stay close together through the whole march.
+// Ray setup: cone_projector_3D_CudaKernel.cu.cc:197-221; march: :87-181
+// (hardware_interp: cone_projector_3D_CudaKernel_hardware_interp.cu.cc:43-142).
+// =====================================================================================
+template <typename TS, bool HW>
+__global__ void __launch_bounds__(256)
+cone_fwd_kernel(const TS *__restrict__ vol, float *__restrict__ sino,
+                const float *__restrict__ inv_ar, const float *__restrict__ src,
+                const float *__restrict__ step_ptr, const float *__restrict__ spacing, int W,
+                int H, int D, int n, int Hd, int Wd, int tiles_x, long long vol_stride) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tile = blockIdx.x;
+  const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+  const int col = tx * 16 + (wave & 1) * 8 + (lane & 7);
+  const int row = ty * 16 + (wave >> 1) * 8 + (lane >> 3);
+  const int view = blockIdx.y, b = blockIdx.z;
+  if (col >= Wd || row >= Hd) return;
+
+  const size_t vb = (size_t)b * n + view;
+  const float *m = inv_ar + vb * 9;
+  const float s0 = src[vb * 3 + 0], s1 = src[vb * 3 + 1], s2 = src[vb * 3 + 2];
+  const float step = step_ptr[b];
+  const float spx = spacing[b * 3 + 2], spy = spacing[b * 3 + 1], spz = spacing[b * 3 + 0];
+  const TS *v = vol + (size_t)b * vol_stride;
+  const int Wp = W + kApronExtra, HWp = (H + kApronExtra) * Wp;
+  const float fW = (float)W, fH = (float)H, fD = (float)D;
+  float *out = sino + (vb * Hd + row) * (size_t)Wd + col;
+
+  const float u = (float)col, w = (float)row;
+  float rx = fmaf(u, m[0], fmaf(w, m[1], m[2]));
+  float ry = fmaf(u, m[3], fmaf(w, m[4], m[5]));
+  float rz = fmaf(u, m[6], fmaf(w, m[7], m[8]));
+  const float nrm = 1.0f / sqrtf(fmaf(rz, rz, fmaf(ry, ry, rx * rx)));
+  rx *= nrm;
+  ry *= nrm;
+  rz *= nrm;
+
+  if (!(step > 0.0f)) {
+    *out = __builtin_nanf("");
+    return;
+  }
+
+  // Slab clipping (cone_projector_3D_CudaKernel.cu.cc:96-132; texture box :53-79).
+  const float lo = HW ? -0.5f : 0.0f;
+  const float hx = HW ? fW - 0.5f : fW, hy = HW ? fH - 0.5f : fH, hz = HW ? fD - 0.5f : fD;
+  float amin = 0.0f, amax = __builtin_inff();
+  if (0.0f != rx) {
+    const float reci = 1.0f / rx;
+    const float a0 = (lo - s0) * reci, a1 = (hx - s0) * reci;
+    amin = fminf(a0, a1);
+    amax = fmaxf(a0, a1);
+  }
+  if (0.0f != ry) {
+    const float reci = 1.0f / ry;
+    const float a0 = (lo - s1) * reci, a1 = (hy - s1) * reci;
+    amin = fmaxf(amin, fminf(a0, a1));
+    amax = fminf(amax, fmaxf(a0, a1));
+  }
+  if (0.0f != rz) {
+    const float reci = 1.0f / rz;
+    const float a0 = (lo - s2) * reci, a1 = (hz - s2) * reci;
+    amin = fmaxf(amin, fminf(a0, a1));
+    amax = fminf(amax, fmaxf(a0, a1));
+  }
+  amin = fmaf(step, 0.5f, amin);
+
+  float pix = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
+  const int cap = march_cap(amin, amax, step);
+  if (amin < amax) {
+    px = fmaf(amin, rx, s0);
+    py = fmaf(amin, ry, s1);
+    pz = fmaf(amin, rz, s2);
+    pix = fmaf(0.5f, cone_sample<TS, HW>(v, Wp, HWp, px, py, pz, fW, fH, fD), pix);
+    amin += step;
+  }
+  for (int it = 0; amin < amax && it < cap; ++it) {
+    px = fmaf(amin, rx, s0);
+    py = fmaf(amin, ry, s1);
+    pz = fmaf(amin, rz, s2);
+    pix += cone_sample<TS, HW>(v, Wp, HWp, px, py, pz, fW, fH, fD);
+    amin += step;
+  }
+  pix *= step;
+  if (pix > 0.0f) {  // tail only for a positive sum (cone_projector_3D_CudaKernel.cu.cc:164-179)
+    const float flast = cone_sample<TS, HW>(v, Wp, HWp, px, py, pz, fW, fH, fD);
+    pix = fmaf(-(0.5f * step), flast, pix);
+    amin -= step;
+    const float last = amax - amin;
+    pix = fmaf(0.5f * last, flast, pix);
+    px = fmaf(amax, rx, s0);
+    py = fmaf(amax, ry, s1);
+    pz = fmaf(amax, rz, s2);
+    pix = fmaf(0.5f * last, cone_sample<TS, HW>(v, Wp, HWp, px, py, pz, fW, fH, fD), pix);
+  }
+  const float a = rx * spx, bq = ry * spy, c = rz * spz;
+  *out = pix * sqrtf(fmaf(c, c, fmaf(bq, bq, a * a)));
+}
+
+// =====================================================================================
+// Backprojection.  A 256-lane block covers 16 x 16 voxels in (x, y); each lane
+// accumulates ZPT consecutive z voxels, sharing the x/y part of P x between them
+// (the products are formed in the reference's order m0 x, + m1 y, + m2 z, + m3, so
+// sharing the first two terms changes nothing).  P rows are wave-uniform and come
+// through scalar loads.  cone_backprojector_3D_CudaKernel.cu.cc:44-53 (map),
+// :84-118 (kernel); texture variant cone_backprojector_3D_CudaKernel_hardware_interp.cu.cc:55-87.
+// =====================================================================================
+template <typename TS, bool HW>
+__device__ __forceinline__ float sample2(const TS *__restrict__ img, int Wp, float u, float v,
+                                         float fW, float fH) {
+  if (HW) {  // tex2DLayered(u + 0.5, v + 0.5, n)
+    u = (u + 0.5f) - 0.5f;
+    v = (v + 0.5f) - 0.5f;
+  }
+  u = clamp_apron(u, fW);
+  v = clamp_apron(v, fH);
+  const float fx = floorf(u), fy = floorf(v);
+  const float xd = u - fx, yd = v - fy;
+  const int base = ((int)fy + kApronLo) * Wp + ((int)fx + kApronLo);
+  const float p00 = to_f32(img[base]), p01 = to_f32(img[base + 1]);
+  const float p10 = to_f32(img[base + Wp]), p11 = to_f32(img[base + Wp + 1]);
+  const float p0 = lerp_ref(p00, p01, xd);
+  const float p1 = lerp_ref(p10, p11, xd);
+  return lerp_ref(p0, p1, yd);
+}
+
+template <typename TS, bool HW, int ZPT>
+__global__ void __launch_bounds__(256)
+cone_bp_kernel(const TS *__restrict__ sino, float *__restrict__ vol, const float *__restrict__ P,
+               const float *__restrict__ mult, const float *__restrict__ spacing,
+               const float *__restrict__ origin, int W, int H, int D, int n, int Hd, int Wd,
+               int zblocks) {
+  const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int b = blockIdx.z / zblocks;
+  const int k0 = (blockIdx.z - b * zblocks) * ZPT;
+  if (i >= W || j >= H) return;
+
+  const float sx = spacing[b * 3 + 2], sy = spacing[b * 3 + 1], sz = spacing[b * 3 + 0];
+  const float ox = origin[b * 3 + 2], oy = origin[b * 3 + 1], oz = origin[b * 3 + 0];
+  const float x = fmaf((float)i, sx, ox), y = fmaf((float)j, sy, oy);
+  float z[ZPT], acc[ZPT];
+#pragma unroll
+  for (int q = 0; q < ZPT; ++q) {
+    z[q] = fmaf((float)(k0 + q), sz, oz);
+    acc[q] = 0.0f;
+  }
+  const int Wp = Wd + kApronExtra;
+  const size_t img_stride = (size_t)(Hd + kApronExtra) * Wp;
+  const TS *img = sino + (size_t)b * n * img_stride;
+  const float *Pb = P + (size_t)b * n * 12;
+  const float fW = (float)Wd, fH = (float)Hd;
+
+  for (int view = 0; view < n; ++view, img += img_stride) {
+    const float *m = Pb + view * 12;
+    const float tu = fmaf(m[1], y, m[0] * x);
+    const float tv = fmaf(m[5], y, m[4] * x);
+    const float tw = fmaf(m[9], y, m[8] * x);
+#pragma unroll
+    for (int q = 0; q < ZPT; ++q) {
+      float uu = fmaf(m[2], z[q], tu) + m[3];
+      float vv = fmaf(m[6], z[q], tv) + m[7];
+      float ww = fmaf(m[10], z[q], tw) + m[11];
+      ww = 1.0f / ww;
+      uu *= ww;
+      vv *= ww;
+      const float f = sample2<TS, HW>(img, Wp, uu, vv, fW, fH);
+      acc[q] = fmaf(f * ww, ww, acc[q]);
+    }
+  }
+  const float mu = mult[b];
+#pragma unroll
+  for (int q = 0; q < ZPT; ++q) {
+    if (k0 + q < D) vol[(((size_t)b * D + k0 + q) * H + j) * W + i] = acc[q] * mu;
+  }
+}
+
+// =====================================================================================
+// Host side
+// =====================================================================================
+static size_t padded_volume_elems(int D, int H, int W) {
+  return (size_t)(D + kApronExtra) * (H + kApronExtra) * (W + kApronExtra);
+}
+static size_t padded_sino_elems(int n, int Hd, int Wd) {
+  return (size_t)n * (Hd + kApronExtra) * (Wd + kApronExtra);
+}
+
+template <typename TS>
+static void launch_apron(const void *src, void *dst, int outer, int Z, int H, int W, int zx, int zlo,
+                         hipStream_t stream) {
+  const long long total = (long long)outer * (Z + zx) * (H + kApronExtra) * (W + kApronExtra);
+  long long blocks = (total + 255) / 256;
+  if (blocks > 65536 * 4) blocks = 65536 * 4;
+  apron_copy_kernel<TS><<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(
+      (const TS *)src, (TS *)dst, Z, H, W, zx, zlo, total);
+}
+
+static void apron(int dtype, const void *src, void *dst, int outer, int Z, int H, int W, int zx,
+                  int zlo, hipStream_t stream) {
+  if (dtype == PYRONN_F32) launch_apron<float>(src, dst, outer, Z, H, W, zx, zlo, stream);
+  else if (dtype == PYRONN_BF16) launch_apron<bf16s>(src, dst, outer, Z, H, W, zx, zlo, stream);
+  else launch_apron<_Float16>(src, dst, outer, Z, H, W, zx, zlo, stream);
+}
+
+template <typename TS>
+static void launch_fwd(bool hw, const void *vol, float *sino, const float *inv_ar, const float *src,
+                       const float *step, const float *spacing, int B, int D, int H, int W, int n,
+                       int Hd, int Wd, hipStream_t stream) {
+  const int tiles_x = (Wd + 15) / 16, tiles_y = (Hd + 15) / 16;
+  dim3 grid(tiles_x * tiles_y, n, B);
+  const long long vs = (long long)padded_volume_elems(D, H, W);
+  if (hw)
+    cone_fwd_kernel<TS, true><<<grid, dim3(256), 0, stream>>>(
+        (const TS *)vol, sino, inv_ar, src, step, spacing, W, H, D, n, Hd, Wd, tiles_x, vs);
+  else
+    cone_fwd_kernel<TS, false><<<grid, dim3(256), 0, stream>>>(
+        (const TS *)vol, sino, inv_ar, src, step, spacing, W, H, D, n, Hd, Wd, tiles_x, vs);
+}
+
+constexpr int kBpZ = 4;
+
+template <typename TS>
+static void launch_bp(bool hw, const void *sino, float *vol, const float *P, const float *mult,
+                      const float *spacing, const float *origin, int B, int D, int H, int W, int n,
+                      int Hd, int Wd, hipStream_t stream) {
+  const int zblocks = (D + kBpZ - 1) / kBpZ;
+  dim3 grid((W + 15) / 16, (H + 15) / 16, zblocks * B);
+  if (hw)
+    cone_bp_kernel<TS, true, kBpZ><<<grid, dim3(256), 0, stream>>>(
+        (const TS *)sino, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd, zblocks);
+  else
+    cone_bp_kernel<TS, false, kBpZ><<<grid, dim3(256), 0, stream>>>(
+        (const TS *)sino, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd, zblocks);
+}
+
+}  // namespace pyronn
+
+using namespace pyronn;
+
+extern "C" int pyronn_cone_geometry_setup(const float *projection_matrices,
+                                          const float *volume_origin, const float *volume_spacing,
+                                          int batch, int n_views, float *inv_ar, float *src,
+                                          pyronn_stream_t stream) {
+  clear_error();
+  PYRONN_CHECK_ARG(batch > 0 && n_views > 0, "cone_geometry_setup: batch=%d n_views=%d", batch,
+                   n_views);
+  PYRONN_CHECK_ARG(projection_matrices && volume_origin && volume_spacing && inv_ar && src,
+                   "cone_geometry_setup: null pointer");
+  const int total = batch * n_views;
+  cone_setup_kernel<<<dim3((total + 127) / 128), dim3(128), 0, (hipStream_t)stream>>>(
+      projection_matrices, volume_origin, volume_spacing, batch, n_views, inv_ar, src);
+  PYRONN_CHECK_LAUNCH("cone_setup_kernel");
+  return PYRONN_OK;
+}
+
+static size_t fwd_geom_bytes(int batch, int n) { return align_up((size_t)batch * n * 12 * 4, 256); }
+
+extern "C" size_t pyronn_cone_projection3d_workspace_size(int batch, int D, int H, int W,
+                                                          int n_views, int volume_dtype) {
+  if (batch <= 0 || D <= 0 || H <= 0 || W <= 0 || n_views <= 0 || !dtype_ok(volume_dtype)) return 0;
+  return fwd_geom_bytes(batch, n_views) +
+         align_up((size_t)batch * padded_volume_elems(D, H, W) * dtype_bytes(volume_dtype), 256);
+}
+
+extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, int batch, int D,
+                                        int H, int W, const float *volume_origin,
+                                        const float *volume_spacing,
+                                        const float *projection_matrices, const float *step_size,
+                                        const float *projection_multiplier, int n_views, int det_h,
+                                        int det_w, int hardware_interp, float *sinogram,
+                                        void *workspace, size_t workspace_bytes,
+                                        pyronn_stream_t stream) {
+  (void)projection_multiplier;  // read but unused by the reference (cone_projector_3D_OPKernel.cc:213-215)
+  clear_error();
+  PYRONN_CHECK_ARG(volume && volume_origin && volume_spacing && projection_matrices && step_size &&
+                       sinogram && workspace,
+                   "cone_projection3d: null pointer");
+  PYRONN_CHECK_ARG(dtype_ok(volume_dtype), "cone_projection3d: bad volume dtype %d", volume_dtype);
+  PYRONN_CHECK_ARG(batch > 0 && batch <= 65535 && D > 0 && H > 0 && W > 0 && n_views > 0 &&
+                       n_views <= 65535 && det_h > 0 && det_w > 0,
+                   "cone_projection3d: bad shape B=%d D=%d H=%d W=%d n=%d det=%dx%d", batch, D, H, W,
+                   n_views, det_h, det_w);
+  if (padded_volume_elems(D, H, W) >= (size_t)INT32_MAX) {
+    set_error("cone_projection3d: volume %dx%dx%d exceeds the 2^31-element limit", D, H, W);
+    return PYRONN_ERR_TOO_LARGE;
+  }
+  const size_t need = pyronn_cone_projection3d_workspace_size(batch, D, H, W, n_views, volume_dtype);
+  if (workspace_bytes < need) {
+    set_error("cone_projection3d: workspace %zu bytes < required %zu", workspace_bytes, need);
+    return PYRONN_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  char *ws = (char *)workspace;
+  float *inv_ar = (float *)ws;
+  float *srcp = inv_ar + (size_t)batch * n_views * 9;
+  void *volp = ws + fwd_geom_bytes(batch, n_views);
+
+  int rc = pyronn_cone_geometry_setup(projection_matrices, volume_origin, volume_spacing, batch,
+                                      n_views, inv_ar, srcp, stream);
+  if (rc != PYRONN_OK) return rc;
+  apron(volume_dtype, volume, volp, batch, D, H, W, kApronExtra, kApronLo, s);
+  PYRONN_CHECK_LAUNCH("apron_copy_kernel(volume)");
+  const bool hw = hardware_interp != 0;
+  if (volume_dtype == PYRONN_F32)
+    launch_fwd<float>(hw, volp, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H, W,
+                      n_views, det_h, det_w, s);
+  else if (volume_dtype == PYRONN_BF16)
+    launch_fwd<bf16s>(hw, volp, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H, W,
+                      n_views, det_h, det_w, s);
+  else
+    launch_fwd<_Float16>(hw, volp, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H,
+                         W, n_views, det_h, det_w, s);
+  PYRONN_CHECK_LAUNCH("cone_fwd_kernel");
+  return PYRONN_OK;
+}
+
+extern "C" size_t pyronn_cone_backprojection3d_workspace_size(int batch, int n_views, int det_h,
+                                                              int det_w, int sinogram_dtype) {
+  if (batch <= 0 || n_views <= 0 || det_h <= 0 || det_w <= 0 || !dtype_ok(sinogram_dtype)) return 0;
+  return align_up((size_t)batch * padded_sino_elems(n_views, det_h, det_w) *
+                      dtype_bytes(sinogram_dtype),
+                  256);
+}
+
+extern "C" int pyronn_cone_backprojection3d(const void *sinogram, int sinogram_dtype, int batch,
+                                            int n_views, int det_h, int det_w,
+                                            const float *volume_origin,
+                                            const float *volume_spacing,
+                                            const float *projection_matrices,
+                                            const float *step_size,
+                                            const float *projection_multiplier, int D, int H,
+                                            int W, int hardware_interp, float *volume,
+                                            void *workspace, size_t workspace_bytes,
+                                            pyronn_stream_t stream) {
+  (void)step_size;  // read but unused by the reference (cone_backprojector_3D_OPKernel.cc:99-101)
+  clear_error();
+  PYRONN_CHECK_ARG(sinogram && volume_origin && volume_spacing && projection_matrices &&
+                       projection_multiplier && volume && workspace,
+                   "cone_backprojection3d: null pointer");
+  PYRONN_CHECK_ARG(dtype_ok(sinogram_dtype), "cone_backprojection3d: bad sinogram dtype %d",
+                   sinogram_dtype);
+  PYRONN_CHECK_ARG(batch > 0 && D > 0 && H > 0 && W > 0 && n_views > 0 && det_h > 0 && det_w > 0,
+                   "cone_backprojection3d: bad shape B=%d D=%d H=%d W=%d n=%d det=%dx%d", batch, D,
+                   H, W, n_views, det_h, det_w);
+  const int zblocks = (D + kBpZ - 1) / kBpZ;
+  PYRONN_CHECK_ARG((long long)zblocks * batch <= 65535,
+                   "cone_backprojection3d: batch x depth too large for one launch");
+  if ((size_t)(det_h + kApronExtra) * (det_w + kApronExtra) >= (size_t)INT32_MAX) {
+    set_error("cone_backprojection3d: detector too large");
+    return PYRONN_ERR_TOO_LARGE;
+  }
+  const size_t need =
+      pyronn_cone_backprojection3d_workspace_size(batch, n_views, det_h, det_w, sinogram_dtype);
+  if (workspace_bytes < need) {
+    set_error("cone_backprojection3d: workspace %zu bytes < required %zu", workspace_bytes, need);
+    return PYRONN_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  apron(sinogram_dtype, sinogram, workspace, batch * n_views, 1, det_h, det_w, 0, 0, s);
+  PYRONN_CHECK_LAUNCH("apron_copy_kernel(sinogram)");
+  const bool hw = hardware_interp != 0;
+  if (sinogram_dtype == PYRONN_F32)
+    launch_bp<float>(hw, workspace, volume, projection_matrices, projection_multiplier,
+                     volume_spacing, volume_origin, batch, D, H, W, n_views, det_h, det_w, s);
+  else if (sinogram_dtype == PYRONN_BF16)
+    launch_bp<bf16s>(hw, workspace, volume, projection_matrices, projection_multiplier,
+                     volume_spacing, volume_origin, batch, D, H, W, n_views, det_h, det_w, s);
+  else
+    launch_bp<_Float16>(hw, workspace, volume, projection_matrices, projection_multiplier,
+                        volume_spacing, volume_origin, batch, D, H, W, n_views, det_h, det_w, s);
+  PYRONN_CHECK_LAUNCH("cone_bp_kernel");
+  return PYRONN_OK;
+}

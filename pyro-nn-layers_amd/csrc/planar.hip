@@ -1,0 +1,326 @@
+// Parallel- and fan-beam 2D projector / backprojector for gfx950.
+//
+// The reference runs all four 2D kernels on the CUDA texture unit (linear
+// filter, border addressing, unnormalised coordinates).  gfx950 has no image
+// sampler, so the texture fetch is done in software: the coordinate c handed
+// to the sampler is formed exactly as the reference forms it
+// (physical_to_index(p) + 0.5f), then b = c - 0.5f, i = floor(b), weights
+// b - i (exact, not 8-bit), neighbours outside the array read 0.
+//
+// The 2D problems are small (a 512^2 image is 1 MiB), so the samplers test
+// bounds per neighbour instead of reading a zero-apron copy.
+#include "common.h"
+
+namespace pyronn {
+
+constexpr float kPi = 3.14159265359f;  // par_backprojector_2D_CudaKernel.cu.cc:32
+
+// helper_grid.h:23-31
+__device__ __forceinline__ float i2p(float i, float o, float s) { return fmaf(i, s, o); }
+__device__ __forceinline__ float p2i(float p, float o, float s) { return (p - o) / s; }
+
+template <typename TS>
+__device__ __forceinline__ float tex2d(const TS *__restrict__ img, int W, int H, float cx,
+                                       float cy) {
+  float bx = cx - 0.5f, by = cy - 0.5f;
+  if (!(bx > -2.0f && bx < (float)(W + 1) && by > -2.0f && by < (float)(H + 1))) return 0.0f;
+  const float fx = floorf(bx), fy = floorf(by);
+  const int x0 = (int)fx, y0 = (int)fy, x1 = x0 + 1, y1 = y0 + 1;
+  const bool vx0 = x0 >= 0 && x0 < W, vx1 = x1 >= 0 && x1 < W;
+  const bool vy0 = y0 >= 0 && y0 < H, vy1 = y1 >= 0 && y1 < H;
+  const float p00 = (vx0 && vy0) ? to_f32(img[y0 * W + x0]) : 0.0f;
+  const float p01 = (vx1 && vy0) ? to_f32(img[y0 * W + x1]) : 0.0f;
+  const float p10 = (vx0 && vy1) ? to_f32(img[y1 * W + x0]) : 0.0f;
+  const float p11 = (vx1 && vy1) ? to_f32(img[y1 * W + x1]) : 0.0f;
+  const float xd = bx - fx, yd = by - fy;
+  const float p0 = lerp_ref(p00, p01, xd);
+  const float p1 = lerp_ref(p10, p11, xd);
+  return lerp_ref(p0, p1, yd);
+}
+
+template <typename TS>
+__device__ __forceinline__ float tex1d(const TS *__restrict__ row, int W, float cx) {
+  const float bx = cx - 0.5f;
+  if (!(bx > -2.0f && bx < (float)(W + 1))) return 0.0f;
+  const float fx = floorf(bx);
+  const int x0 = (int)fx, x1 = x0 + 1;
+  const float p0 = (x0 >= 0 && x0 < W) ? to_f32(row[x0]) : 0.0f;
+  const float p1 = (x1 >= 0 && x1 < W) ? to_f32(row[x1]) : 0.0f;
+  return lerp_ref(p0, p1, bx - fx);
+}
+
+// kernel_project2D, par_projector_2D_CudaKernel.cu.cc:28-102 (same code in
+// fan_projector_2D_CudaKernel.cu.cc:27-102): march in mm, box edges with the
+// -0.5 mm offset, first sample exactly at the entry point, tail for a positive sum.
+template <typename TS>
+__device__ float march2d(const TS *__restrict__ img, int W, int H, float ox, float oy, float sx,
+                         float sy, float s0, float s1, float r0, float r1, float step) {
+  if (!(step > 0.0f)) return __builtin_nanf("");
+  float amin = 0.0f, amax = __builtin_inff();
+  if (0.0f != r0) {
+    const float e0 = i2p(0.0f, ox, sx) - 0.5f, e1 = i2p((float)W, ox, sx) - 0.5f;
+    const float reci = 1.0f / r0;
+    const float a0 = (e0 - s0) * reci, a1 = (e1 - s0) * reci;
+    amin = fminf(a0, a1);
+    amax = fmaxf(a0, a1);
+  }
+  if (0.0f != r1) {
+    const float e0 = i2p(0.0f, oy, sy) - 0.5f, e1 = i2p((float)H, oy, sy) - 0.5f;
+    const float reci = 1.0f / r1;
+    const float a0 = (e0 - s1) * reci, a1 = (e1 - s1) * reci;
+    amin = fmaxf(amin, fminf(a0, a1));
+    amax = fminf(amax, fmaxf(a0, a1));
+  }
+  float pix = 0.0f, px = 0.0f, py = 0.0f;
+  const int cap = march_cap(amin, amax, step);
+#define PYRONN_TEX2D(X, Y) tex2d(img, W, H, p2i((X), ox, sx) + 0.5f, p2i((Y), oy, sy) + 0.5f)
+  if (amin < amax) {
+    px = fmaf(amin, r0, s0);
+    py = fmaf(amin, r1, s1);
+    pix = fmaf(0.5f, PYRONN_TEX2D(px, py), pix);
+    amin += step;
+  }
+  for (int it = 0; amin < amax && it < cap; ++it) {
+    px = fmaf(amin, r0, s0);
+    py = fmaf(amin, r1, s1);
+    pix += PYRONN_TEX2D(px, py);
+    amin += step;
+  }
+  pix *= step;
+  if (pix > 0.0f) {
+    const float flast = PYRONN_TEX2D(px, py);
+    pix = fmaf(-(0.5f * step), flast, pix);
+    amin -= step;
+    const float last = amax - amin;
+    pix = fmaf(0.5f * last, flast, pix);
+    px = fmaf(amax, r0, s0);
+    py = fmaf(amax, r1, s1);
+    pix = fmaf(0.5f * last, PYRONN_TEX2D(px, py), pix);
+  }
+#undef PYRONN_TEX2D
+  return pix;
+}
+
+// One lane per (detector pixel, view, batch item).
+// Parallel: project_2Dpar_beam_kernel, par_projector_2D_CudaKernel.cu.cc:104-147.
+// Fan:      project_2Dfan_beam_kernel, fan_projector_2D_CudaKernel.cu.cc:104-149.
+template <typename TS, bool FAN>
+__global__ void __launch_bounds__(256)
+proj2d_kernel(const TS *__restrict__ vol, float *__restrict__ sino, const float *__restrict__ rays,
+              const float *__restrict__ vorigin, const float *__restrict__ dorigin,
+              const float *__restrict__ vspacing, const float *__restrict__ dspacing,
+              const float *__restrict__ sid_p, const float *__restrict__ sdd_p, int H, int W, int n,
+              int det, float step_factor) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  const int view = blockIdx.y, b = blockIdx.z;
+  if (d >= det) return;
+  const float sx = vspacing[b * 2 + 1], sy = vspacing[b * 2 + 0];
+  const float ox = vorigin[b * 2 + 1], oy = vorigin[b * 2 + 0];
+  const size_t vb = (size_t)b * n + view;
+  const float c0 = rays[vb * 2 + 0], c1 = rays[vb * 2 + 1];
+  const float u0 = -c1, u1 = c0;
+  const float u = i2p((float)d, dorigin[b], dspacing[b]);
+  float s0, s1, r0, r1;
+  if (!FAN) {
+    const float a = (((float)W * sx) * (float)W) * sx;
+    const float sid = sqrtf(fmaf(((float)H * sy) * (float)H, sy, a)) * 1.2f;
+    s0 = fmaf(u0, u, c0 * (-sid));
+    s1 = fmaf(u1, u, c1 * (-sid));
+    r0 = c0;
+    r1 = c1;
+  } else {
+    const float sid = sid_p[b], sdd = sdd_p[b];
+    s0 = c0 * (-sid);
+    s1 = c1 * (-sid);
+    const float dx = fmaf(u0, u, fmaf(c0, sdd, s0));
+    const float dy = fmaf(u1, u, fmaf(c1, sdd, s1));
+    const float ex = dx - s0, ey = dy - s1;
+    const float nrm = 1.0f / sqrtf(fmaf(ey, ey, ex * ex));
+    r0 = ex * nrm;
+    r1 = ey * nrm;
+  }
+  const float step = step_factor * fminf(sx, sy);
+  const TS *img = vol + (size_t)b * H * W;
+  float pix = march2d(img, W, H, ox, oy, sx, sy, s0, s1, r0, r1, step);
+  const float a = r0 * sx, c = r1 * sy;
+  sino[vb * det + d] = pix * sqrtf(fmaf(c, c, a * a));
+}
+
+// 2D line intersection, helper_geometry_gpu.h:28-44 (NaN when |den| < 1e-6).
+__device__ __forceinline__ void isect2(float p1x, float p1y, float p2x, float p2y, float p3x,
+                                       float p3y, float p4x, float p4y, float &ox, float &oy) {
+  const float a = p1x - p2x, b = p3y - p4y, c = p1y - p2y, d = p3x - p4x;
+  const float den = fmaf(-c, d, a * b);
+  if (den < 0.000001f && den > -0.000001f) {
+    ox = oy = __builtin_nanf("");
+    return;
+  }
+  const float e = fmaf(-p1y, p2x, p1x * p2y);
+  const float g = fmaf(-p3y, p4x, p3x * p4y);
+  ox = fmaf(-a, g, e * d) / den;
+  oy = fmaf(-c, g, e * b) / den;
+}
+
+// One lane per pixel; loop over all views.
+// Parallel: backproject_2Dpar_beam_kernel, par_backprojector_2D_CudaKernel.cu.cc:28-61.
+// Fan:      backproject_2Dfan_beam_kernel, fan_backprojector_2D_CudaKernel.cu.cc:29-67.
+template <typename TS, bool FAN>
+__global__ void __launch_bounds__(256)
+back2d_kernel(const TS *__restrict__ sino, float *__restrict__ vol, const float *__restrict__ rays,
+              const float *__restrict__ vorigin, const float *__restrict__ dorigin,
+              const float *__restrict__ vspacing, const float *__restrict__ dspacing,
+              const float *__restrict__ sid_p, const float *__restrict__ sdd_p, int H, int W, int n,
+              int det) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int b = blockIdx.z;
+  if (x >= W || y >= H) return;
+  const float sx = vspacing[b * 2 + 1], sy = vspacing[b * 2 + 0];
+  const float ox = vorigin[b * 2 + 1], oy = vorigin[b * 2 + 0];
+  const float do_ = dorigin[b], ds = dspacing[b];
+  const float pcx = i2p((float)x, ox, sx), pcy = i2p((float)y, oy, sy);
+  const float *rb = rays + (size_t)b * n * 2;
+  const TS *sb = sino + (size_t)b * n * det;
+  float val = 0.0f;
+  if (!FAN) {
+    for (int v = 0; v < n; ++v) {
+      const float c0 = rb[v * 2 + 0], c1 = rb[v * 2 + 1];
+      const float s = fmaf(pcy, c0, pcx * (-c1));
+      const float si = p2i(s, do_, ds);
+      val += tex1d(sb + (size_t)v * det, det, si + 0.5f);
+    }
+    vol[((size_t)b * H + y) * W + x] = (kPi * val) / (float)n;
+  } else {
+    const float sid = sid_p[b], sdd = sdd_p[b];
+    for (int v = 0; v < n; ++v) {
+      const float c0 = rb[v * 2 + 0], c1 = rb[v * 2 + 1];
+      const float d0 = -c1, d1 = c0;
+      const float srx = c0 * (-sid), sry = c1 * (-sid);
+      const float cpx = fmaf(c0, sdd, srx), cpy = fmaf(c1, sdd, sry);
+      float ix, iy;
+      isect2(pcx, pcy, srx, sry, cpx, cpy, cpx + d0, cpy + d1, ix, iy);
+      const float ex = pcx - srx, ey = pcy - sry;
+      const float dw = 1.0f / sqrtf(fmaf(ey, ey, ex * ex));
+      const float s = fmaf(iy, d1, ix * d0);
+      const float si = p2i(s, do_, ds);
+      const float f = tex1d(sb + (size_t)v * det, det, si + 0.5f);
+      val = fmaf(f * dw, dw, val);
+    }
+    vol[((size_t)b * H + y) * W + x] = (((sid * sdd) * kPi) * val) / (float)n;
+  }
+}
+
+template <bool FAN>
+static int launch_proj2d(const char *name, const void *volume, int dtype, int batch, int H, int W,
+                         const float *vo, const float *dor, const float *vs, const float *ds,
+                         const float *sid, const float *sdd, const float *rays, int n, int det,
+                         float step_factor, float *sino, hipStream_t s) {
+  clear_error();
+  PYRONN_CHECK_ARG(volume && vo && dor && vs && ds && rays && sino, "%s: null pointer", name);
+  PYRONN_CHECK_ARG(!FAN || (sid && sdd), "%s: null source distances", name);
+  PYRONN_CHECK_ARG(dtype_ok(dtype), "%s: bad dtype %d", name, dtype);
+  PYRONN_CHECK_ARG(batch > 0 && batch <= 65535 && H > 0 && W > 0 && n > 0 && n <= 65535 && det > 0,
+                   "%s: bad shape B=%d H=%d W=%d n=%d det=%d", name, batch, H, W, n, det);
+  PYRONN_CHECK_ARG((long long)H * W < INT32_MAX, "%s: image too large", name);
+  PYRONN_CHECK_ARG(step_factor > 0.0f, "%s: step factor must be > 0", name);
+  dim3 grid((det + 255) / 256, n, batch);
+  if (dtype == PYRONN_F32)
+    proj2d_kernel<float, FAN><<<grid, dim3(256), 0, s>>>((const float *)volume, sino, rays, vo, dor, vs,
+                                                         ds, sid, sdd, H, W, n, det, step_factor);
+  else if (dtype == PYRONN_BF16)
+    proj2d_kernel<bf16s, FAN><<<grid, dim3(256), 0, s>>>((const bf16s *)volume, sino, rays, vo, dor, vs,
+                                                         ds, sid, sdd, H, W, n, det, step_factor);
+  else
+    proj2d_kernel<_Float16, FAN><<<grid, dim3(256), 0, s>>>(
+        (const _Float16 *)volume, sino, rays, vo, dor, vs, ds, sid, sdd, H, W, n, det, step_factor);
+  PYRONN_CHECK_LAUNCH(name);
+  return PYRONN_OK;
+}
+
+template <bool FAN>
+static int launch_back2d(const char *name, const void *sino, int dtype, int batch, int n, int det,
+                         const float *vo, const float *dor, const float *vs, const float *ds,
+                         const float *sid, const float *sdd, const float *rays, int H, int W,
+                         float *vol, hipStream_t s) {
+  clear_error();
+  PYRONN_CHECK_ARG(sino && vo && dor && vs && ds && rays && vol, "%s: null pointer", name);
+  PYRONN_CHECK_ARG(!FAN || (sid && sdd), "%s: null source distances", name);
+  PYRONN_CHECK_ARG(dtype_ok(dtype), "%s: bad dtype %d", name, dtype);
+  PYRONN_CHECK_ARG(batch > 0 && batch <= 65535 && H > 0 && W > 0 && n > 0 && det > 0,
+                   "%s: bad shape B=%d H=%d W=%d n=%d det=%d", name, batch, H, W, n, det);
+  dim3 grid((W + 15) / 16, (H + 15) / 16, batch);
+  if (dtype == PYRONN_F32)
+    back2d_kernel<float, FAN><<<grid, dim3(256), 0, s>>>((const float *)sino, vol, rays, vo, dor, vs, ds,
+                                                         sid, sdd, H, W, n, det);
+  else if (dtype == PYRONN_BF16)
+    back2d_kernel<bf16s, FAN><<<grid, dim3(256), 0, s>>>((const bf16s *)sino, vol, rays, vo, dor, vs, ds,
+                                                         sid, sdd, H, W, n, det);
+  else
+    back2d_kernel<_Float16, FAN><<<grid, dim3(256), 0, s>>>((const _Float16 *)sino, vol, rays, vo, dor,
+                                                            vs, ds, sid, sdd, H, W, n, det);
+  PYRONN_CHECK_LAUNCH(name);
+  return PYRONN_OK;
+}
+
+}  // namespace pyronn
+
+using namespace pyronn;
+
+extern "C" int pyronn_parallel_projection2d(const void *volume, int volume_dtype, int batch, int H,
+                                            int W, const float *volume_origin,
+                                            const float *detector_origin,
+                                            const float *volume_spacing,
+                                            const float *detector_spacing,
+                                            const float *ray_vectors, int n_views, int det,
+                                            float step_factor, float *sinogram,
+                                            pyronn_stream_t stream) {
+  return launch_proj2d<false>("parallel_projection2d", volume, volume_dtype, batch, H, W,
+                              volume_origin, detector_origin, volume_spacing, detector_spacing,
+                              nullptr, nullptr, ray_vectors, n_views, det, step_factor, sinogram,
+                              (hipStream_t)stream);
+}
+
+extern "C" int pyronn_fan_projection2d(const void *volume, int volume_dtype, int batch, int H,
+                                       int W, const float *volume_origin,
+                                       const float *detector_origin, const float *volume_spacing,
+                                       const float *detector_spacing,
+                                       const float *source_2_isocenter_distance,
+                                       const float *source_2_detector_distance,
+                                       const float *central_ray_vectors, int n_views, int det,
+                                       float step_factor, float *sinogram,
+                                       pyronn_stream_t stream) {
+  return launch_proj2d<true>("fan_projection2d", volume, volume_dtype, batch, H, W, volume_origin,
+                             detector_origin, volume_spacing, detector_spacing,
+                             source_2_isocenter_distance, source_2_detector_distance,
+                             central_ray_vectors, n_views, det, step_factor, sinogram,
+                             (hipStream_t)stream);
+}
+
+extern "C" int pyronn_parallel_backprojection2d(const void *sinogram, int sinogram_dtype,
+                                                int batch, int n_views, int det,
+                                                const float *volume_origin,
+                                                const float *detector_origin,
+                                                const float *volume_spacing,
+                                                const float *detector_spacing,
+                                                const float *ray_vectors, int H, int W,
+                                                float *volume, pyronn_stream_t stream) {
+  return launch_back2d<false>("parallel_backprojection2d", sinogram, sinogram_dtype, batch, n_views,
+                              det, volume_origin, detector_origin, volume_spacing,
+                              detector_spacing, nullptr, nullptr, ray_vectors, H, W, volume,
+                              (hipStream_t)stream);
+}
+
+extern "C" int pyronn_fan_backprojection2d(const void *sinogram, int sinogram_dtype, int batch,
+                                           int n_views, int det, const float *volume_origin,
+                                           const float *detector_origin,
+                                           const float *volume_spacing,
+                                           const float *detector_spacing,
+                                           const float *source_2_isocenter_distance,
+                                           const float *source_2_detector_distance,
+                                           const float *central_ray_vectors, int H, int W,
+                                           float *volume, pyronn_stream_t stream) {
+  return launch_back2d<true>("fan_backprojection2d", sinogram, sinogram_dtype, batch, n_views, det,
+                             volume_origin, detector_origin, volume_spacing, detector_spacing,
+                             source_2_isocenter_distance, source_2_detector_distance,
+                             central_ray_vectors, H, W, volume, (hipStream_t)stream);
+}

@@ -1,0 +1,97 @@
+"""C ABI checks that need no GPU: exported symbols, argument validation, op registration."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from pyronn_layers import native as nat
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for f in os.listdir(os.path.join(ROOT, "include")):
+        if f.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", f)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(pyronn_[a-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    declared = _declared()
+    assert len(declared) >= 11
+    L = nat.lib()
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+    assert declared <= set(nat.EXPORTED)
+
+
+def test_version_and_empty_error():
+    assert nat.lib().pyronn_version() >= 100
+    assert nat.last_error() == ""
+
+
+def _fake(n=1):
+    return ctypes.c_void_p(0x1000 * n)
+
+
+def test_invalid_arguments_are_rejected_before_launch():
+    L = nat.lib()
+    # batch = 0
+    rc = L.pyronn_cone_projection3d(_fake(), 0, 0, 4, 4, 4, _fake(), _fake(), _fake(), _fake(), _fake(), 2, 4, 4, 0,
+                                    _fake(), _fake(), 1 << 20, None)
+    assert rc == 1 and "bad shape" in nat.last_error()
+    # bad dtype
+    rc = L.pyronn_cone_projection3d(_fake(), 7, 1, 4, 4, 4, _fake(), _fake(), _fake(), _fake(), _fake(), 2, 4, 4, 0,
+                                    _fake(), _fake(), 1 << 20, None)
+    assert rc == 1 and "dtype" in nat.last_error()
+    # null pointer
+    rc = L.pyronn_cone_backprojection3d(None, 0, 1, 2, 4, 4, _fake(), _fake(), _fake(), _fake(), _fake(), 4, 4, 4,
+                                        0, _fake(), _fake(), 1 << 20, None)
+    assert rc == 1 and "null" in nat.last_error()
+    # workspace too small
+    need = L.pyronn_cone_projection3d_workspace_size(1, 4, 4, 4, 2, 0)
+    assert need > 0
+    rc = L.pyronn_cone_projection3d(_fake(), 0, 1, 4, 4, 4, _fake(), _fake(), _fake(), _fake(), _fake(), 2, 4, 4, 0,
+                                    _fake(), _fake(), need - 1, None)
+    assert rc == 2 and "workspace" in nat.last_error()
+    # 2D: non-positive step factor
+    rc = L.pyronn_parallel_projection2d(_fake(), 0, 1, 8, 8, _fake(), _fake(), _fake(), _fake(), _fake(), 4, 9, 0.0,
+                                        _fake(), None)
+    assert rc == 1 and "step" in nat.last_error()
+    rc = L.pyronn_fan_projection2d(_fake(), 0, 1, 8, 8, _fake(), _fake(), _fake(), _fake(), None, _fake(), _fake(), 4,
+                                   9, 0.2, _fake(), None)
+    assert rc == 1 and "source" in nat.last_error()
+
+
+def test_too_large_volume_rejected():
+    L = nat.lib()
+    rc = L.pyronn_cone_projection3d(_fake(), 0, 1, 2048, 2048, 2048, _fake(), _fake(), _fake(), _fake(), _fake(), 2,
+                                    4, 4, 0, _fake(), _fake(), 1 << 62, None)
+    assert rc == 3
+
+
+def test_workspace_sizes_scale_with_storage():
+    L = nat.lib()
+    f32 = L.pyronn_cone_projection3d_workspace_size(1, 64, 64, 64, 10, 0)
+    bf16 = L.pyronn_cone_projection3d_workspace_size(1, 64, 64, 64, 10, 1)
+    assert f32 > bf16 > 0
+    assert L.pyronn_cone_backprojection3d_workspace_size(2, 10, 8, 8, 0) == \
+        ((2 * 10 * 11 * 11 * 4 + 255) // 256) * 256
+
+
+def test_torch_ops_registered_and_cpu_tensors_rejected():
+    import pyronn_layers as pl
+    pl.load()
+    for name in ["parallel_projection2d", "parallel_backprojection2d", "fan_projection2d",
+                 "fan_backprojection2d", "cone_projection3d", "cone_backprojection3d"]:
+        assert hasattr(torch.ops.pyronn_layers, name)
+    vol = torch.zeros(1, 4, 4, 4)
+    with pytest.raises(Exception):
+        pl.cone_projection3d(vol, [2, 4, 4], torch.zeros(1, 3), torch.ones(1, 3), torch.zeros(1, 2, 3, 4),
+                             torch.ones(1, 1), torch.ones(1, 1))

@@ -1,0 +1,239 @@
+"""GPU parity: HIP kernels (through the C ABI) against the CPU oracle.
+
+FP32 bar (stated tolerance): relative L2 <= 1e-5 and max|diff| <= 1e-4 * max|ref|
+for the same inputs; in practice most elements agree to the last bit.
+BF16 / FP16 storage: the same bar against the oracle fed the dequantised
+input (the arithmetic is FP32 in both), plus relative L2 <= 1e-3 against the
+FP64 oracle on the FP32 input.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from pyronn_layers import geometry as geo
+from pyronn_layers import phantoms as ph
+
+pytestmark = pytest.mark.gpu
+
+gh = pytest.importorskip("gpu_helpers")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+# ----------------------------------------------------------------------------- cone fixtures
+
+@pytest.fixture(scope="module")
+def cone():
+    D, H, W = 48, 44, 40
+    spacing = np.array([1.2, 1.0, 0.9], np.float32)      # z, y, x
+    origin = geo.centred_origin((D, H, W), spacing).astype(np.float32)
+    e = ph.random_ellipsoids(3, 20, (D, H, W), spacing, axes_range_vox=(3, 18))
+    vol = ph.rasterise_ellipsoids(e, (D, H, W), spacing, origin)
+    Hd, Wd = 56, 64
+    P = geo.circular_projection_matrices(24, 150.0, 250.0, (Hd, Wd), (1.4, 1.3), start=0.1).astype(np.float32)
+    inv, src, inv64, src64, st = orc.cone_setup(P, spacing, origin)
+    assert st == 0
+    return dict(vol=vol, spacing=spacing, origin=origin, P=P, det=(Hd, Wd), inv=inv, src=src, inv64=inv64,
+                src64=src64, shape=(D, H, W), step=0.5)
+
+
+def test_geometry_setup_bit_exact(cone):
+    inv, src = gh.cone_setup(cone["P"], cone["spacing"], cone["origin"])
+    np.testing.assert_array_equal(inv[0], cone["inv"])
+    np.testing.assert_array_equal(src[0], cone["src"])
+
+
+def test_geometry_setup_singular_gives_nan():
+    P = np.zeros((1, 2, 3, 4), np.float32)
+    P[0, 0] = geo.circular_projection_matrices(1, 500.0, 900.0, (8, 8), (1.0, 1.0))[0]
+    P[0, 1, :, :3] = [[1, 2, 3], [2, 4, 6], [0, 0, 1]]
+    inv, src = gh.cone_setup(P, np.ones(3, np.float32), np.zeros(3, np.float32))
+    assert np.all(np.isfinite(inv[0, 0])) and np.all(np.isnan(inv[0, 1])) and np.all(np.isnan(src[0, 1]))
+
+
+@pytest.mark.parametrize("hw", [False, True])
+def test_cone_forward_parity(cone, hw):
+    g = gh.cone_forward(cone["vol"], cone["P"], cone["spacing"], cone["origin"], cone["det"], cone["step"], hw=hw)
+    r, _, _ = orc.cone_forward(cone["vol"], cone["inv"], cone["src"], cone["det"], cone["spacing"], cone["step"],
+                               hw=hw)
+    l2, mx, exact = gh.parity(g[0], r)
+    print(f"cone fwd hw={hw}: rel L2 {l2:.2e} max {mx:.2e} bit-exact {exact:.4f}")
+    assert exact > 0.99
+
+
+@pytest.mark.parametrize("hw", [False, True])
+def test_cone_backward_parity(cone, hw):
+    rng = np.random.default_rng(1)
+    sino, _, _ = orc.cone_forward(cone["vol"], cone["inv"], cone["src"], cone["det"], cone["spacing"], 0.5)
+    sino = (sino + rng.normal(0, 0.01 * sino.max(), sino.shape)).astype(np.float32)
+    g = gh.cone_backward(sino, cone["P"], cone["spacing"], cone["origin"], cone["shape"], mult=0.8, hw=hw)
+    r, _ = orc.cone_backward(sino, cone["P"], cone["shape"], cone["spacing"], cone["origin"], mult=0.8, hw=hw)
+    l2, mx, exact = gh.parity(g[0], r)
+    print(f"cone bp hw={hw}: rel L2 {l2:.2e} max {mx:.2e} bit-exact {exact:.4f}")
+    assert exact > 0.99
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_cone_forward_low_precision_storage(cone, storage):
+    g = gh.cone_forward(cone["vol"], cone["P"], cone["spacing"], cone["origin"], cone["det"], cone["step"],
+                        storage=storage)[0]
+    vq = gh.quantise(cone["vol"], storage)
+    r, _, _ = orc.cone_forward(vq, cone["inv"], cone["src"], cone["det"], cone["spacing"], cone["step"])
+    gh.parity(g, r)
+    r64, _ = orc.cone_forward_f64(cone["vol"], cone["inv64"], cone["src64"], cone["det"], cone["spacing"],
+                                  cone["step"])
+    err = np.linalg.norm(g - r64) / np.linalg.norm(r64)
+    print(f"cone fwd {storage}: rel L2 vs FP64 {err:.2e}")
+    assert err <= 1e-3
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_cone_backward_low_precision_storage(cone, storage):
+    sino, _, _ = orc.cone_forward(cone["vol"], cone["inv"], cone["src"], cone["det"], cone["spacing"], 0.5)
+    g = gh.cone_backward(sino, cone["P"], cone["spacing"], cone["origin"], cone["shape"], storage=storage)[0]
+    sq = gh.quantise(sino, storage)
+    r, _ = orc.cone_backward(sq, cone["P"], cone["shape"], cone["spacing"], cone["origin"])
+    gh.parity(g, r)
+    r64 = orc.cone_backward_f64(sino, cone["P"], cone["shape"], cone["spacing"], cone["origin"])
+    err = np.linalg.norm(g - r64) / np.linalg.norm(r64)
+    print(f"cone bp {storage}: rel L2 vs FP64 {err:.2e}")
+    assert err <= 1e-3
+
+
+def test_cone_batch_items_match_single_calls(cone):
+    """Per-item geometry in one launch: each item equals its own single call, bitwise."""
+    vol = cone["vol"]
+    P2 = geo.circular_projection_matrices(24, 170.0, 260.0, cone["det"], (1.4, 1.3), start=0.7).astype(np.float32)
+    vols = np.stack([vol, vol[::-1].copy()])
+    Ps = np.stack([cone["P"], P2])
+    sp = np.stack([cone["spacing"], cone["spacing"] * 1.1])
+    og = np.stack([cone["origin"], cone["origin"] * 1.1])
+    both = gh.cone_forward(vols, Ps, sp, og, cone["det"], [0.5, 0.6])
+    for b in range(2):
+        one = gh.cone_forward(vols[b], Ps[b], sp[b], og[b], cone["det"], [0.5, 0.6][b])
+        np.testing.assert_array_equal(both[b], one[0])
+    bp = gh.cone_backward(both, Ps, sp, og, cone["shape"], mult=np.array([1.0, 2.0]))
+    for b in range(2):
+        one = gh.cone_backward(both[b], Ps[b], sp[b], og[b], cone["shape"], mult=[1.0, 2.0][b])
+        np.testing.assert_array_equal(bp[b], one[0])
+
+
+def test_cone_edge_cases():
+    # 1x1x1 volume, 1x1 detector, a ray that misses, and a non-positive step.
+    vol = np.ones((1, 1, 1), np.float32)
+    sp = np.ones(3, np.float32)
+    og = np.zeros(3, np.float32)
+    P = geo.circular_projection_matrices(2, 10.0, 20.0, (1, 1), (1.0, 1.0)).astype(np.float32)
+    inv, src, _, _, _ = orc.cone_setup(P, sp, og)
+    g = gh.cone_forward(vol, P, sp, og, (1, 1), 0.25)
+    r, _, _ = orc.cone_forward(vol, inv, src, (1, 1), sp, 0.25)
+    np.testing.assert_allclose(g[0], r, rtol=1e-6)
+    assert g.max() > 0
+    # a detector pixel far off-axis misses the 1-voxel volume
+    P3 = geo.circular_projection_matrices(2, 10.0, 20.0, (3, 3), (50.0, 50.0)).astype(np.float32)
+    g3 = gh.cone_forward(vol, P3, sp, og, (3, 3), 0.25)
+    assert g3[0, :, 0, 0].max() == 0.0
+    bad = gh.cone_forward(vol, P, sp, og, (1, 1), 0.0)
+    assert np.all(np.isnan(bad))
+
+
+# ----------------------------------------------------------------------------- 2D
+
+@pytest.fixture(scope="module")
+def par():
+    H = W = 128
+    s = (1.0, 1.0)
+    org = geo.centred_origin((H, W), s).astype(np.float32)
+    e = ph.random_ellipses(21, 12, (H, W), s, axes_range_px=(5, 45))
+    img = ph.rasterise_ellipses(e, (H, W), s, org)
+    n, det, ds = 90, 183, 1.0
+    return dict(img=img, org=org, s=np.array(s, np.float32), rays=geo.parallel_ray_vectors(n).astype(np.float32),
+                det=det, do=np.float32(-(det - 1) / 2 * ds), ds=np.float32(ds), shape=(H, W))
+
+
+@pytest.fixture(scope="module")
+def fan():
+    H = W = 160
+    s = (0.5, 0.5)
+    org = geo.centred_origin((H, W), s).astype(np.float32)
+    e = ph.random_ellipses(22, 16, (H, W), s, axes_range_px=(5, 60))
+    img = ph.rasterise_ellipses(e, (H, W), s, org)
+    n, det, ds = 120, 241, 1.0
+    return dict(img=img, org=org, s=np.array(s, np.float32),
+                rays=geo.fan_central_ray_vectors(n, 2 * math.pi).astype(np.float32), det=det,
+                do=np.float32(-(det - 1) / 2 * ds), ds=np.float32(ds), shape=(H, W), sid=np.float32(1000.0),
+                sdd=np.float32(1200.0))
+
+
+@pytest.mark.parametrize("step", [0.2, 0.5])
+def test_parallel_forward_parity(par, step):
+    g = gh.forward2d(False, par["img"], par["rays"], par["det"], par["org"], par["s"], par["do"], par["ds"],
+                     step_factor=step)[0]
+    r, _, _ = orc.parallel_forward(par["img"], par["rays"], par["det"], par["org"], par["s"], par["do"],
+                                   par["ds"], step_factor=step)
+    l2, mx, exact = gh.parity(g, r)
+    print(f"par fwd step {step}: rel L2 {l2:.2e} max {mx:.2e} bit-exact {exact:.4f}")
+
+
+def test_parallel_backward_parity(par):
+    sino, _, _ = orc.parallel_forward(par["img"], par["rays"], par["det"], par["org"], par["s"], par["do"],
+                                      par["ds"])
+    g = gh.backward2d(False, sino, par["rays"], par["shape"], par["org"], par["s"], par["do"], par["ds"])[0]
+    r, _ = orc.parallel_backward(sino, par["rays"], par["shape"], par["org"], par["s"], par["do"], par["ds"])
+    l2, mx, exact = gh.parity(g, r)
+    print(f"par bp: rel L2 {l2:.2e} max {mx:.2e} bit-exact {exact:.4f}")
+
+
+@pytest.mark.parametrize("step", [0.2, 0.5])
+def test_fan_forward_parity(fan, step):
+    g = gh.forward2d(True, fan["img"], fan["rays"], fan["det"], fan["org"], fan["s"], fan["do"], fan["ds"],
+                     fan["sid"], fan["sdd"], step_factor=step)[0]
+    r, _, _ = orc.fan_forward(fan["img"], fan["rays"], fan["det"], fan["org"], fan["s"], fan["do"], fan["ds"],
+                              fan["sid"], fan["sdd"], step_factor=step)
+    l2, mx, exact = gh.parity(g, r)
+    print(f"fan fwd step {step}: rel L2 {l2:.2e} max {mx:.2e} bit-exact {exact:.4f}")
+
+
+def test_fan_backward_parity(fan):
+    rng = np.random.default_rng(2)
+    sino, _, _ = orc.fan_forward(fan["img"], fan["rays"], fan["det"], fan["org"], fan["s"], fan["do"], fan["ds"],
+                                 fan["sid"], fan["sdd"], step_factor=0.5)
+    sino = (sino + rng.normal(0, 0.01 * sino.max(), sino.shape)).astype(np.float32)
+    g = gh.backward2d(True, sino, fan["rays"], fan["shape"], fan["org"], fan["s"], fan["do"], fan["ds"],
+                      fan["sid"], fan["sdd"])[0]
+    r, _ = orc.fan_backward(sino, fan["rays"], fan["shape"], fan["org"], fan["s"], fan["do"], fan["ds"],
+                            fan["sid"], fan["sdd"])
+    l2, mx, exact = gh.parity(g, r)
+    print(f"fan bp: rel L2 {l2:.2e} max {mx:.2e} bit-exact {exact:.4f}")
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_2d_low_precision_storage(par, storage):
+    g = gh.forward2d(False, par["img"], par["rays"], par["det"], par["org"], par["s"], par["do"], par["ds"],
+                     storage=storage)[0]
+    r, _, _ = orc.parallel_forward(gh.quantise(par["img"], storage), par["rays"], par["det"], par["org"],
+                                   par["s"], par["do"], par["ds"])
+    gh.parity(g, r)
+    sino = r
+    gb = gh.backward2d(False, sino, par["rays"], par["shape"], par["org"], par["s"], par["do"], par["ds"],
+                       storage=storage)[0]
+    rb, _ = orc.parallel_backward(gh.quantise(sino, storage), par["rays"], par["shape"], par["org"], par["s"],
+                                  par["do"], par["ds"])
+    gh.parity(gb, rb)
+
+
+def test_2d_batch_items_match_single_calls(par):
+    imgs = np.stack([par["img"], par["img"].T.copy()])
+    rays2 = geo.parallel_ray_vectors(90, math.pi, start=0.3).astype(np.float32)
+    rays = np.stack([par["rays"], rays2])
+    both = gh.forward2d(False, imgs, rays, par["det"], par["org"], par["s"], par["do"], par["ds"])
+    for b in range(2):
+        one = gh.forward2d(False, imgs[b], rays[b], par["det"], par["org"], par["s"], par["do"], par["ds"])
+        np.testing.assert_array_equal(both[b], one[0])
