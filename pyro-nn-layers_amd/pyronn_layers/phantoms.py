@@ -168,3 +168,35 @@ def ellipse_line_integrals(e: Ellipses, src: np.ndarray, dirs: np.ndarray) -> np
         disc = B * B - 4 * A * C
         out += e.density[k] * np.where(disc > 0, np.sqrt(np.maximum(disc, 0)) / A, 0.0)
     return out
+
+
+def rasterise_ellipsoids_torch(e: Ellipsoids, volume_shape, spacing, origin, device="cuda",
+                               dtype=None, slab: int = 16):
+    """Same rasterisation as rasterise_ellipsoids, computed with torch on `device`
+    (for full-size phantoms).  Accumulates in float64, returns float32."""
+    import torch
+
+    D, H, W = volume_shape
+    sz, sy, sx = spacing
+    oz, oy, ox = origin
+    f64 = torch.float64
+    x = ox + sx * torch.arange(W, device=device, dtype=f64)
+    y = oy + sy * torch.arange(H, device=device, dtype=f64)
+    vol = torch.zeros((D, H, W), device=device, dtype=torch.float32)
+    for z0 in range(0, D, slab):
+        z1 = min(D, z0 + slab)
+        zs = oz + sz * torch.arange(z0, z1, device=device, dtype=f64)
+        Z, Y, X = torch.meshgrid(zs, y, x, indexing="ij")
+        acc = torch.zeros_like(X)
+        for k in range(len(e.density)):
+            R = torch.tensor(e.rot[k], device=device, dtype=f64)
+            a = e.axes[k]
+            c = e.centres[k]
+            dx, dy, dz = X - c[0], Y - c[1], Z - c[2]
+            b0 = dx * R[0, 0] + dy * R[1, 0] + dz * R[2, 0]
+            b1 = dx * R[0, 1] + dy * R[1, 1] + dz * R[2, 1]
+            b2 = dx * R[0, 2] + dy * R[1, 2] + dz * R[2, 2]
+            q = (b0 / a[0]) ** 2 + (b1 / a[1]) ** 2 + (b2 / a[2]) ** 2
+            acc += float(e.density[k]) * (q <= 1.0).to(f64)
+        vol[z0:z1] = acc.to(torch.float32)
+    return vol if dtype is None else vol.to(dtype)
