@@ -54,6 +54,9 @@ def main(argv=None):
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-storage run")
     ap.add_argument("--cpu-views", type=int, default=4, help="forward views in the CPU sample")
     ap.add_argument("--cpu-slices", type=int, default=16, help="z slices in the CPU backprojection sample")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one plain 1 GiB device copy (reference for HBM byte counters)")
+    ap.add_argument("--bf16-only", action="store_true", help="time only the BF16-storage run")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path.",
@@ -70,6 +73,12 @@ def main(argv=None):
 
     pl.load()
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        a = torch.ones(2 ** 28, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     cfg = m1_config()
     D, H, W = cfg["shape"]
     n = cfg["n"]
@@ -120,6 +129,10 @@ def main(argv=None):
         return float(np.mean(times_f)), float(np.mean(times_b)), sino
 
     # Timed region: K steps bracketed by synchronize on both sides (inside run()).
+    if args.bf16_only:
+        tf, tb, _ = run(torch.bfloat16)
+        log(f"bf16 only: fwd {tf * 1e3:.1f} ms, bp {tb * 1e3:.1f} ms")
+        return 0
     torch.cuda.synchronize()
     t_wall0 = time.perf_counter()
     tf, tb, sino = run(torch.float32)

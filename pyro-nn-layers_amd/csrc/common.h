@@ -7,11 +7,11 @@
 //  * Division and square root are IEEE correctly rounded
 //    (-fhip-fp32-correctly-rounded-divide-sqrt); rsqrtf(x) of the reference is
 //    evaluated as 1.0f / sqrtf(x).
-//  * Samplers clamp the coordinate into [-1, N] before the floor and read from
-//    a copy of the data surrounded by a zero apron (1 element before, 2 after
-//    each interpolated axis).  For every finite coordinate this is exactly the
-//    reference's "zero outside [0, N-1]" rule without a bounds test per
-//    neighbour; NaN coordinates read zeros.
+//  * The cone samplers clamp the coordinate into [-1, N] before the floor and
+//    read neighbourhood cells built over index range [-1, N] with zeros outside
+//    the data.  For every finite coordinate this is exactly the reference's
+//    "zero outside [0, N-1]" rule without a bounds test per neighbour; NaN
+//    coordinates read zeros.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -61,10 +61,6 @@ __device__ __forceinline__ int march_cap(float amin, float amax, float step) {
   const float n = ceilf(1.25f * (amax - amin) / step) + 16.0f;
   return n < 4194304.0f ? (int)n : 4194304;
 }
-
-// Sizes of the zero-apron copies.
-constexpr int kApronLo = 1;
-constexpr int kApronExtra = 3;  // 1 before + 2 after
 
 // ----------------------------------------------------------------- host errors
 void set_error(const char *fmt, ...);

@@ -2,8 +2,10 @@
 //
 // Kernels
 //   cone_setup_kernel   per-view FP64 geometry (M^-1, camera centre) on the device
-//   apron_copy_kernel   zero-apron copy (and storage-type passthrough) of the
-//                       volume / sinogram that the samplers read
+//   build_cells_kernel  copy of the volume as 2x2x2 neighbourhood cells with a zero
+//                       border (what the forward sampler reads)
+//   build_det_cells_kernel  copy of the sinogram as 2x2 neighbourhood cells with a
+//                       zero border (what the backprojection sampler reads)
 //   cone_fwd_kernel     ray-driven forward projection, one lane per detector pixel
 //   cone_bp_kernel      voxel-driven backprojection, one lane per ZPT voxels in z
 //
@@ -13,6 +15,8 @@
 #include "common.h"
 
 namespace pyronn {
+
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
 
 // =====================================================================================
 // Per-view geometry (replaces cone_projector_3D_OPKernel.cc:96-182 and
@@ -57,46 +61,86 @@ __global__ void cone_setup_kernel(const float *__restrict__ P, const float *__re
 }
 
 // =====================================================================================
-// Zero-apron copy: src [outer][Z][H][W] -> dst [outer][Z + zx][H + 3][W + 3], data at
-// offset (zlo, 1, 1).  For the volume zx = 3, zlo = 1 (z is interpolated); for the
-// sinogram zx = 0 (views are not).
+// Neighbourhood cells.  The forward projector reads the volume through a copy in
+// which cell (x, y, z), x in [-1, W], y in [-1, H], z in [-1, D], holds the eight
+// voxels of its trilinear neighbourhood
+//   [v(x,y,z), v(x+1,y,z), v(x,y+1,z), v(x+1,y+1,z), (same at z+1)]
+// with zeros outside the volume.  One sample then costs one 16-byte load (bf16 /
+// f16 storage) or two (f32) instead of eight 4-byte gathers, and the zero border
+// makes the reference's per-neighbour bounds tests unnecessary.
 // =====================================================================================
 template <typename TS>
-__global__ void apron_copy_kernel(const TS *__restrict__ src, TS *__restrict__ dst, int Z, int H,
-                                  int W, int zx, int zlo, long long total) {
-  const int Wp = W + kApronExtra, Hp = H + kApronExtra, Zp = Z + zx;
+__device__ __forceinline__ TS vox(const TS *__restrict__ v, int x, int y, int z, int W, int H,
+                                  int D) {
+  if (x < 0 || x >= W || y < 0 || y >= H || z < 0 || z >= D) return zero_of<TS>();
+  return v[((size_t)z * H + y) * W + x];
+}
+
+template <typename TS>
+__global__ void build_cells_kernel(const TS *__restrict__ vol, TS *__restrict__ cells, int W,
+                                   int H, int D, long long total) {
+  const int Wc = W + 2, Hc = H + 2, Dc = D + 2;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
     long long r = t;
-    const int xp = (int)(r % Wp); r /= Wp;
-    const int yp = (int)(r % Hp); r /= Hp;
-    const int zp = (int)(r % Zp);
-    const long long o = r / Zp;
-    const int x = xp - kApronLo, y = yp - kApronLo, z = zp - zlo;
-    TS v = zero_of<TS>();
-    if (x >= 0 && x < W && y >= 0 && y < H && z >= 0 && z < Z)
-      v = src[((o * Z + z) * (long long)H + y) * W + x];
-    dst[t] = v;
+    const int cx = (int)(r % Wc); r /= Wc;
+    const int cy = (int)(r % Hc); r /= Hc;
+    const int cz = (int)(r % Dc);
+    const long long b = r / Dc;
+    const TS *v = vol + (size_t)b * W * H * D;
+    const int x = cx - 1, y = cy - 1, z = cz - 1;
+    TS c[8];
+    c[0] = vox(v, x, y, z, W, H, D);
+    c[1] = vox(v, x + 1, y, z, W, H, D);
+    c[2] = vox(v, x, y + 1, z, W, H, D);
+    c[3] = vox(v, x + 1, y + 1, z, W, H, D);
+    c[4] = vox(v, x, y, z + 1, W, H, D);
+    c[5] = vox(v, x + 1, y, z + 1, W, H, D);
+    c[6] = vox(v, x, y + 1, z + 1, W, H, D);
+    c[7] = vox(v, x + 1, y + 1, z + 1, W, H, D);
+    TS *o = cells + (size_t)t * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) o[q] = c[q];
   }
 }
 
-// =====================================================================================
-// Trilinear sample from the zero-apron volume, lerp z -> y -> x
-// (cone_projector_3D_CudaKernel.cu.cc:42-85).
-// =====================================================================================
+__device__ __forceinline__ void unpack2(uint32_t w, bf16s, float &a, float &b) {
+  a = __uint_as_float(w << 16);
+  b = __uint_as_float(w & 0xffff0000u);
+}
+__device__ __forceinline__ void unpack2(uint32_t w, _Float16, float &a, float &b) {
+  _Float16 h[2];
+  __builtin_memcpy(h, &w, 4);
+  a = (float)h[0];
+  b = (float)h[1];
+}
+
+// Trilinear sample at (px, py, pz), lerp z -> y -> x exactly as
+// cone_projector_3D_CudaKernel.cu.cc:74-82.  The coordinate is clamped into the
+// zero border first (identical results for every finite coordinate).
 template <typename TS>
-__device__ __forceinline__ float sample3(const TS *__restrict__ v, int Wp, int HWp, float px,
-                                         float py, float pz, float fW, float fH, float fD) {
+__device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int Wc, int HWc,
+                                              float px, float py, float pz, float fW, float fH,
+                                              float fD) {
   px = clamp_apron(px, fW);
   py = clamp_apron(py, fH);
   pz = clamp_apron(pz, fD);
   const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
   const float xd = px - fx, yd = py - fy, zd = pz - fz;
-  const int base = ((int)fz + kApronLo) * HWp + ((int)fy + kApronLo) * Wp + ((int)fx + kApronLo);
-  const float p000 = to_f32(v[base]), p001 = to_f32(v[base + 1]);
-  const float p010 = to_f32(v[base + Wp]), p011 = to_f32(v[base + Wp + 1]);
-  const float p100 = to_f32(v[base + HWp]), p101 = to_f32(v[base + HWp + 1]);
-  const float p110 = to_f32(v[base + HWp + Wp]), p111 = to_f32(v[base + HWp + Wp + 1]);
+  const unsigned cell = (unsigned)(((int)fz + 1) * HWc + ((int)fy + 1) * Wc + ((int)fx + 1));
+  float p000, p001, p010, p011, p100, p101, p110, p111;
+  if constexpr (sizeof(TS) == 4) {
+    const float4 *c4 = reinterpret_cast<const float4 *>(cells) + (size_t)cell * 2;
+    const float4 a = c4[0], b = c4[1];
+    p000 = a.x; p001 = a.y; p010 = a.z; p011 = a.w;
+    p100 = b.x; p101 = b.y; p110 = b.z; p111 = b.w;
+  } else {
+    const uint4 q = reinterpret_cast<const uint4 *>(cells)[cell];
+    unpack2(q.x, TS{}, p000, p001);
+    unpack2(q.y, TS{}, p010, p011);
+    unpack2(q.z, TS{}, p100, p101);
+    unpack2(q.w, TS{}, p110, p111);
+  }
   const float p00 = lerp_ref(p000, p100, zd);
   const float p01 = lerp_ref(p001, p101, zd);
   const float p10 = lerp_ref(p010, p110, zd);
@@ -107,35 +151,50 @@ __device__ __forceinline__ float sample3(const TS *__restrict__ v, int Wp, int H
 }
 
 template <typename TS, bool HW>
-__device__ __forceinline__ float cone_sample(const TS *__restrict__ v, int Wp, int HWp, float px,
+__device__ __forceinline__ float cone_sample(const TS *__restrict__ c, int Wc, int HWc, float px,
                                              float py, float pz, float fW, float fH, float fD) {
   if (HW) {  // tex3D(p + 0.5): the sampler subtracts the half texel again
     px = (px + 0.5f) - 0.5f;
     py = (py + 0.5f) - 0.5f;
     pz = (pz + 0.5f) - 0.5f;
   }
-  return sample3(v, Wp, HWp, px, py, pz, fW, fH, fD);
+  return sample_cells(c, Wc, HWc, px, py, pz, fW, fH, fD);
 }
 
 // =====================================================================================
 // Forward projection.  One lane per (view, row, col); a 256-lane block covers a
 // 16 x 16 detector tile and each wave an 8 x 8 sub-tile, so the 64 rays of a wave
 // stay close together through the whole march.
+// Work order: workgroups are dispatched round-robin over the 8 XCDs, so block id
+// b runs on XCD b % 8.  XCD k takes detector tile rows k, k+8, k+16, ...; within a
+// tile row it walks views (outer) and tiles along the row (inner).  The blocks
+// resident on one XCD at any time are therefore a few neighbouring views of one
+// 16-row band of the detector: their rays cross the same z slab of the volume,
+// nearly coincide from view to view, and share that XCD's L2.  The order changes
+// only which lane computes which ray, never the arithmetic of a ray.
 // Ray setup: cone_projector_3D_CudaKernel.cu.cc:197-221; march: :87-181
 // (hardware_interp: cone_projector_3D_CudaKernel_hardware_interp.cu.cc:43-142).
 // =====================================================================================
 template <typename TS, bool HW>
 __global__ void __launch_bounds__(256)
-cone_fwd_kernel(const TS *__restrict__ vol, float *__restrict__ sino,
+cone_fwd_kernel(const TS *__restrict__ cells, float *__restrict__ sino,
                 const float *__restrict__ inv_ar, const float *__restrict__ src,
                 const float *__restrict__ step_ptr, const float *__restrict__ spacing, int W,
-                int H, int D, int n, int Hd, int Wd, int tiles_x, long long vol_stride) {
+                int H, int D, int n, int Hd, int Wd, int tiles_x, int tiles_y,
+                long long cell_stride) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tile = blockIdx.x;
-  const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+  const int xcd = (int)(blockIdx.x % kXcds);
+  const long long q = blockIdx.x / kXcds;
+  const long long per_row = (long long)n * tiles_x;
+  const int band = (int)(q / per_row);
+  const int ty = band * kXcds + xcd;
+  if (ty >= tiles_y) return;
+  const long long w_ = q - (long long)band * per_row;
+  const int view = (int)(w_ / tiles_x);
+  const int tx = (int)(w_ - (long long)view * tiles_x);
   const int col = tx * 16 + (wave & 1) * 8 + (lane & 7);
   const int row = ty * 16 + (wave >> 1) * 8 + (lane >> 3);
-  const int view = blockIdx.y, b = blockIdx.z;
+  const int b = blockIdx.z;
   if (col >= Wd || row >= Hd) return;
 
   const size_t vb = (size_t)b * n + view;
@@ -143,8 +202,8 @@ cone_fwd_kernel(const TS *__restrict__ vol, float *__restrict__ sino,
   const float s0 = src[vb * 3 + 0], s1 = src[vb * 3 + 1], s2 = src[vb * 3 + 2];
   const float step = step_ptr[b];
   const float spx = spacing[b * 3 + 2], spy = spacing[b * 3 + 1], spz = spacing[b * 3 + 0];
-  const TS *v = vol + (size_t)b * vol_stride;
-  const int Wp = W + kApronExtra, HWp = (H + kApronExtra) * Wp;
+  const TS *v = cells + (size_t)b * cell_stride * 8;
+  const int Wp = W + 2, HWp = (H + 2) * Wp;
   const float fW = (float)W, fH = (float)H, fD = (float)D;
   float *out = sino + (vb * Hd + row) * (size_t)Wd + col;
 
@@ -219,16 +278,46 @@ cone_fwd_kernel(const TS *__restrict__ vol, float *__restrict__ sino,
 }
 
 // =====================================================================================
-// Backprojection.  A 256-lane block covers 16 x 16 voxels in (x, y); each lane
-// accumulates ZPT consecutive z voxels, sharing the x/y part of P x between them
-// (the products are formed in the reference's order m0 x, + m1 y, + m2 z, + m3, so
-// sharing the first two terms changes nothing).  P rows are wave-uniform and come
-// through scalar loads.  cone_backprojector_3D_CudaKernel.cu.cc:44-53 (map),
-// :84-118 (kernel); texture variant cone_backprojector_3D_CudaKernel_hardware_interp.cu.cc:55-87.
+// Detector cells for the backprojection: per view, cell (u, v), u in [-1, W_d],
+// v in [-1, H_d], holds the bilinear neighbourhood
+//   [s(u,v), s(u+1,v), s(u,v+1), s(u+1,v+1)]
+// with zeros outside the detector, so one update is one 16-byte (f32) or 8-byte
+// (bf16 / f16) load.
 // =====================================================================================
+template <typename TS>
+__device__ __forceinline__ TS pix(const TS *__restrict__ s, int u, int v, int Wd, int Hd) {
+  if (u < 0 || u >= Wd || v < 0 || v >= Hd) return zero_of<TS>();
+  return s[(size_t)v * Wd + u];
+}
+
+template <typename TS>
+__global__ void build_det_cells_kernel(const TS *__restrict__ sino, TS *__restrict__ cells,
+                                       int Hd, int Wd, long long total) {
+  const int Wc = Wd + 2, Hc = Hd + 2;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    long long r = t;
+    const int cu = (int)(r % Wc); r /= Wc;
+    const int cv = (int)(r % Hc);
+    const long long img = r / Hc;
+    const TS *s = sino + (size_t)img * Hd * Wd;
+    const int u = cu - 1, v = cv - 1;
+    TS c[4];
+    c[0] = pix(s, u, v, Wd, Hd);
+    c[1] = pix(s, u + 1, v, Wd, Hd);
+    c[2] = pix(s, u, v + 1, Wd, Hd);
+    c[3] = pix(s, u + 1, v + 1, Wd, Hd);
+    TS *o = cells + (size_t)t * 4;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) o[q] = c[q];
+  }
+}
+
+// Bilinear sample, lerp x then y (cone_backprojector_3D_CudaKernel.cu.cc:55-82),
+// zero outside the detector.
 template <typename TS, bool HW>
-__device__ __forceinline__ float sample2(const TS *__restrict__ img, int Wp, float u, float v,
-                                         float fW, float fH) {
+__device__ __forceinline__ float sample_det(const TS *__restrict__ img, int Wc, float u, float v,
+                                            float fW, float fH) {
   if (HW) {  // tex2DLayered(u + 0.5, v + 0.5, n)
     u = (u + 0.5f) - 0.5f;
     v = (v + 0.5f) - 0.5f;
@@ -237,17 +326,32 @@ __device__ __forceinline__ float sample2(const TS *__restrict__ img, int Wp, flo
   v = clamp_apron(v, fH);
   const float fx = floorf(u), fy = floorf(v);
   const float xd = u - fx, yd = v - fy;
-  const int base = ((int)fy + kApronLo) * Wp + ((int)fx + kApronLo);
-  const float p00 = to_f32(img[base]), p01 = to_f32(img[base + 1]);
-  const float p10 = to_f32(img[base + Wp]), p11 = to_f32(img[base + Wp + 1]);
+  const unsigned c = (unsigned)(((int)fy + 1) * Wc + ((int)fx + 1));
+  float p00, p01, p10, p11;
+  if constexpr (sizeof(TS) == 4) {
+    const float4 a = reinterpret_cast<const float4 *>(img)[c];
+    p00 = a.x; p01 = a.y; p10 = a.z; p11 = a.w;
+  } else {
+    const uint2 a = reinterpret_cast<const uint2 *>(img)[c];
+    unpack2(a.x, TS{}, p00, p01);
+    unpack2(a.y, TS{}, p10, p11);
+  }
   const float p0 = lerp_ref(p00, p01, xd);
   const float p1 = lerp_ref(p10, p11, xd);
   return lerp_ref(p0, p1, yd);
 }
 
+// =====================================================================================
+// Backprojection.  A 256-lane block covers 16 x 16 voxels in (x, y); each lane
+// accumulates ZPT consecutive z voxels, sharing the x/y part of P x between them
+// (the products are formed in the reference's order m0 x, + m1 y, + m2 z, + m3, so
+// sharing the first two terms changes nothing).  P rows are wave-uniform and come
+// through scalar loads.  cone_backprojector_3D_CudaKernel.cu.cc:44-53 (map),
+// :84-118 (kernel); texture variant cone_backprojector_3D_CudaKernel_hardware_interp.cu.cc:55-87.
+// =====================================================================================
 template <typename TS, bool HW, int ZPT>
 __global__ void __launch_bounds__(256)
-cone_bp_kernel(const TS *__restrict__ sino, float *__restrict__ vol, const float *__restrict__ P,
+cone_bp_kernel(const TS *__restrict__ cells, float *__restrict__ vol, const float *__restrict__ P,
                const float *__restrict__ mult, const float *__restrict__ spacing,
                const float *__restrict__ origin, int W, int H, int D, int n, int Hd, int Wd,
                int zblocks) {
@@ -266,9 +370,9 @@ cone_bp_kernel(const TS *__restrict__ sino, float *__restrict__ vol, const float
     z[q] = fmaf((float)(k0 + q), sz, oz);
     acc[q] = 0.0f;
   }
-  const int Wp = Wd + kApronExtra;
-  const size_t img_stride = (size_t)(Hd + kApronExtra) * Wp;
-  const TS *img = sino + (size_t)b * n * img_stride;
+  const int Wc = Wd + 2;
+  const size_t img_stride = (size_t)(Hd + 2) * Wc * 4;
+  const TS *img = cells + (size_t)b * n * img_stride;
   const float *Pb = P + (size_t)b * n * 12;
   const float fW = (float)Wd, fH = (float)Hd;
 
@@ -285,7 +389,7 @@ cone_bp_kernel(const TS *__restrict__ sino, float *__restrict__ vol, const float
       ww = 1.0f / ww;
       uu *= ww;
       vv *= ww;
-      const float f = sample2<TS, HW>(img, Wp, uu, vv, fW, fH);
+      const float f = sample_det<TS, HW>(img, Wc, uu, vv, fW, fH);
       acc[q] = fmaf(f * ww, ww, acc[q]);
     }
   }
@@ -299,59 +403,70 @@ cone_bp_kernel(const TS *__restrict__ sino, float *__restrict__ vol, const float
 // =====================================================================================
 // Host side
 // =====================================================================================
-static size_t padded_volume_elems(int D, int H, int W) {
-  return (size_t)(D + kApronExtra) * (H + kApronExtra) * (W + kApronExtra);
+static size_t volume_cells(int D, int H, int W) {
+  return (size_t)(D + 2) * (H + 2) * (W + 2);
 }
-static size_t padded_sino_elems(int n, int Hd, int Wd) {
-  return (size_t)n * (Hd + kApronExtra) * (Wd + kApronExtra);
+static size_t det_cells(int n, int Hd, int Wd) { return (size_t)n * (Hd + 2) * (Wd + 2); }
+
+// Blocks of one forward launch per batch item: 8 XCD lanes x tile-row bands x views x tiles.
+static long long fwd_blocks(int n, int Hd, int Wd) {
+  const long long tiles_x = (Wd + 15) / 16, tiles_y = (Hd + 15) / 16;
+  return (long long)kXcds * ((tiles_y + kXcds - 1) / kXcds) * n * tiles_x;
 }
 
 template <typename TS>
-static void launch_apron(const void *src, void *dst, int outer, int Z, int H, int W, int zx, int zlo,
+static void launch_cells(const void *vol, void *cells, int B, int D, int H, int W,
                          hipStream_t stream) {
-  const long long total = (long long)outer * (Z + zx) * (H + kApronExtra) * (W + kApronExtra);
+  const long long total = (long long)B * volume_cells(D, H, W);
   long long blocks = (total + 255) / 256;
   if (blocks > 65536 * 4) blocks = 65536 * 4;
-  apron_copy_kernel<TS><<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(
-      (const TS *)src, (TS *)dst, Z, H, W, zx, zlo, total);
-}
-
-static void apron(int dtype, const void *src, void *dst, int outer, int Z, int H, int W, int zx,
-                  int zlo, hipStream_t stream) {
-  if (dtype == PYRONN_F32) launch_apron<float>(src, dst, outer, Z, H, W, zx, zlo, stream);
-  else if (dtype == PYRONN_BF16) launch_apron<bf16s>(src, dst, outer, Z, H, W, zx, zlo, stream);
-  else launch_apron<_Float16>(src, dst, outer, Z, H, W, zx, zlo, stream);
+  build_cells_kernel<TS><<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(
+      (const TS *)vol, (TS *)cells, W, H, D, total);
 }
 
 template <typename TS>
-static void launch_fwd(bool hw, const void *vol, float *sino, const float *inv_ar, const float *src,
-                       const float *step, const float *spacing, int B, int D, int H, int W, int n,
-                       int Hd, int Wd, hipStream_t stream) {
+static void launch_fwd(bool hw, const void *cells, float *sino, const float *inv_ar,
+                       const float *src, const float *step, const float *spacing, int B, int D,
+                       int H, int W, int n, int Hd, int Wd, hipStream_t stream) {
   const int tiles_x = (Wd + 15) / 16, tiles_y = (Hd + 15) / 16;
-  dim3 grid(tiles_x * tiles_y, n, B);
-  const long long vs = (long long)padded_volume_elems(D, H, W);
+  const long long blocks = fwd_blocks(n, Hd, Wd);
+  dim3 grid((unsigned)blocks, 1, B);
+  const long long cs = (long long)volume_cells(D, H, W);
   if (hw)
     cone_fwd_kernel<TS, true><<<grid, dim3(256), 0, stream>>>(
-        (const TS *)vol, sino, inv_ar, src, step, spacing, W, H, D, n, Hd, Wd, tiles_x, vs);
+        (const TS *)cells, sino, inv_ar, src, step, spacing, W, H, D, n, Hd, Wd, tiles_x, tiles_y, cs);
   else
     cone_fwd_kernel<TS, false><<<grid, dim3(256), 0, stream>>>(
-        (const TS *)vol, sino, inv_ar, src, step, spacing, W, H, D, n, Hd, Wd, tiles_x, vs);
+        (const TS *)cells, sino, inv_ar, src, step, spacing, W, H, D, n, Hd, Wd, tiles_x, tiles_y, cs);
 }
 
-constexpr int kBpZ = 4;
+// z voxels per lane: 8 for f32 detector cells, 4 for the 16-bit ones (measured
+// on the 512^3 / 720-view / 1024x768 workload).
+template <typename TS> constexpr int bp_zpt() { return sizeof(TS) == 4 ? 8 : 4; }
 
 template <typename TS>
-static void launch_bp(bool hw, const void *sino, float *vol, const float *P, const float *mult,
+static void launch_det_cells(const void *sino, void *cells, int images, int Hd, int Wd,
+                             hipStream_t stream) {
+  const long long total = (long long)images * (Hd + 2) * (Wd + 2);
+  long long blocks = (total + 255) / 256;
+  if (blocks > 65536 * 4) blocks = 65536 * 4;
+  build_det_cells_kernel<TS><<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(
+      (const TS *)sino, (TS *)cells, Hd, Wd, total);
+}
+
+template <typename TS>
+static void launch_bp(bool hw, const void *cells, float *vol, const float *P, const float *mult,
                       const float *spacing, const float *origin, int B, int D, int H, int W, int n,
                       int Hd, int Wd, hipStream_t stream) {
-  const int zblocks = (D + kBpZ - 1) / kBpZ;
+  constexpr int Z = bp_zpt<TS>();
+  const int zblocks = (D + Z - 1) / Z;
   dim3 grid((W + 15) / 16, (H + 15) / 16, zblocks * B);
   if (hw)
-    cone_bp_kernel<TS, true, kBpZ><<<grid, dim3(256), 0, stream>>>(
-        (const TS *)sino, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd, zblocks);
+    cone_bp_kernel<TS, true, Z><<<grid, dim3(256), 0, stream>>>(
+        (const TS *)cells, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd, zblocks);
   else
-    cone_bp_kernel<TS, false, kBpZ><<<grid, dim3(256), 0, stream>>>(
-        (const TS *)sino, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd, zblocks);
+    cone_bp_kernel<TS, false, Z><<<grid, dim3(256), 0, stream>>>(
+        (const TS *)cells, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd, zblocks);
 }
 
 }  // namespace pyronn
@@ -380,7 +495,7 @@ extern "C" size_t pyronn_cone_projection3d_workspace_size(int batch, int D, int 
                                                           int n_views, int volume_dtype) {
   if (batch <= 0 || D <= 0 || H <= 0 || W <= 0 || n_views <= 0 || !dtype_ok(volume_dtype)) return 0;
   return fwd_geom_bytes(batch, n_views) +
-         align_up((size_t)batch * padded_volume_elems(D, H, W) * dtype_bytes(volume_dtype), 256);
+         align_up((size_t)batch * volume_cells(D, H, W) * 8 * dtype_bytes(volume_dtype), 256);
 }
 
 extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, int batch, int D,
@@ -401,8 +516,12 @@ extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, in
                        n_views <= 65535 && det_h > 0 && det_w > 0,
                    "cone_projection3d: bad shape B=%d D=%d H=%d W=%d n=%d det=%dx%d", batch, D, H, W,
                    n_views, det_h, det_w);
-  if (padded_volume_elems(D, H, W) >= (size_t)INT32_MAX) {
-    set_error("cone_projection3d: volume %dx%dx%d exceeds the 2^31-element limit", D, H, W);
+  if (volume_cells(D, H, W) >= (size_t)INT32_MAX) {
+    set_error("cone_projection3d: volume %dx%dx%d exceeds the 2^31-cell limit", D, H, W);
+    return PYRONN_ERR_TOO_LARGE;
+  }
+  if (fwd_blocks(n_views, det_h, det_w) >= (long long)INT32_MAX) {
+    set_error("cone_projection3d: %d views of %dx%d pixels exceed one launch", n_views, det_h, det_w);
     return PYRONN_ERR_TOO_LARGE;
   }
   const size_t need = pyronn_cone_projection3d_workspace_size(batch, D, H, W, n_views, volume_dtype);
@@ -414,23 +533,25 @@ extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, in
   char *ws = (char *)workspace;
   float *inv_ar = (float *)ws;
   float *srcp = inv_ar + (size_t)batch * n_views * 9;
-  void *volp = ws + fwd_geom_bytes(batch, n_views);
+  void *cells = ws + fwd_geom_bytes(batch, n_views);
 
   int rc = pyronn_cone_geometry_setup(projection_matrices, volume_origin, volume_spacing, batch,
                                       n_views, inv_ar, srcp, stream);
   if (rc != PYRONN_OK) return rc;
-  apron(volume_dtype, volume, volp, batch, D, H, W, kApronExtra, kApronLo, s);
-  PYRONN_CHECK_LAUNCH("apron_copy_kernel(volume)");
   const bool hw = hardware_interp != 0;
-  if (volume_dtype == PYRONN_F32)
-    launch_fwd<float>(hw, volp, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H, W,
+  if (volume_dtype == PYRONN_F32) {
+    launch_cells<float>(volume, cells, batch, D, H, W, s);
+    launch_fwd<float>(hw, cells, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H, W,
                       n_views, det_h, det_w, s);
-  else if (volume_dtype == PYRONN_BF16)
-    launch_fwd<bf16s>(hw, volp, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H, W,
+  } else if (volume_dtype == PYRONN_BF16) {
+    launch_cells<bf16s>(volume, cells, batch, D, H, W, s);
+    launch_fwd<bf16s>(hw, cells, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H, W,
                       n_views, det_h, det_w, s);
-  else
-    launch_fwd<_Float16>(hw, volp, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H,
+  } else {
+    launch_cells<_Float16>(volume, cells, batch, D, H, W, s);
+    launch_fwd<_Float16>(hw, cells, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H,
                          W, n_views, det_h, det_w, s);
+  }
   PYRONN_CHECK_LAUNCH("cone_fwd_kernel");
   return PYRONN_OK;
 }
@@ -438,8 +559,7 @@ extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, in
 extern "C" size_t pyronn_cone_backprojection3d_workspace_size(int batch, int n_views, int det_h,
                                                               int det_w, int sinogram_dtype) {
   if (batch <= 0 || n_views <= 0 || det_h <= 0 || det_w <= 0 || !dtype_ok(sinogram_dtype)) return 0;
-  return align_up((size_t)batch * padded_sino_elems(n_views, det_h, det_w) *
-                      dtype_bytes(sinogram_dtype),
+  return align_up((size_t)batch * det_cells(n_views, det_h, det_w) * 4 * dtype_bytes(sinogram_dtype),
                   256);
 }
 
@@ -463,10 +583,9 @@ extern "C" int pyronn_cone_backprojection3d(const void *sinogram, int sinogram_d
   PYRONN_CHECK_ARG(batch > 0 && D > 0 && H > 0 && W > 0 && n_views > 0 && det_h > 0 && det_w > 0,
                    "cone_backprojection3d: bad shape B=%d D=%d H=%d W=%d n=%d det=%dx%d", batch, D,
                    H, W, n_views, det_h, det_w);
-  const int zblocks = (D + kBpZ - 1) / kBpZ;
-  PYRONN_CHECK_ARG((long long)zblocks * batch <= 65535,
+  PYRONN_CHECK_ARG((long long)((D + 3) / 4) * batch <= 65535,
                    "cone_backprojection3d: batch x depth too large for one launch");
-  if ((size_t)(det_h + kApronExtra) * (det_w + kApronExtra) >= (size_t)INT32_MAX) {
+  if ((size_t)(det_h + 2) * (det_w + 2) >= (size_t)INT32_MAX / 4) {
     set_error("cone_backprojection3d: detector too large");
     return PYRONN_ERR_TOO_LARGE;
   }
@@ -477,18 +596,21 @@ extern "C" int pyronn_cone_backprojection3d(const void *sinogram, int sinogram_d
     return PYRONN_ERR_WORKSPACE;
   }
   hipStream_t s = (hipStream_t)stream;
-  apron(sinogram_dtype, sinogram, workspace, batch * n_views, 1, det_h, det_w, 0, 0, s);
-  PYRONN_CHECK_LAUNCH("apron_copy_kernel(sinogram)");
   const bool hw = hardware_interp != 0;
-  if (sinogram_dtype == PYRONN_F32)
+  const int images = batch * n_views;
+  if (sinogram_dtype == PYRONN_F32) {
+    launch_det_cells<float>(sinogram, workspace, images, det_h, det_w, s);
     launch_bp<float>(hw, workspace, volume, projection_matrices, projection_multiplier,
                      volume_spacing, volume_origin, batch, D, H, W, n_views, det_h, det_w, s);
-  else if (sinogram_dtype == PYRONN_BF16)
+  } else if (sinogram_dtype == PYRONN_BF16) {
+    launch_det_cells<bf16s>(sinogram, workspace, images, det_h, det_w, s);
     launch_bp<bf16s>(hw, workspace, volume, projection_matrices, projection_multiplier,
                      volume_spacing, volume_origin, batch, D, H, W, n_views, det_h, det_w, s);
-  else
+  } else {
+    launch_det_cells<_Float16>(sinogram, workspace, images, det_h, det_w, s);
     launch_bp<_Float16>(hw, workspace, volume, projection_matrices, projection_multiplier,
                         volume_spacing, volume_origin, batch, D, H, W, n_views, det_h, det_w, s);
+  }
   PYRONN_CHECK_LAUNCH("cone_bp_kernel");
   return PYRONN_OK;
 }

@@ -32,35 +32,34 @@ def cone_forward_sample_count(P, volume_shape, spacing_zyx, origin_zyx, detector
     vv, uu = torch.meshgrid(torch.arange(Hd, device=dev, dtype=f64), torch.arange(Wd, device=dev, dtype=f64),
                             indexing="ij")
     pix = torch.stack([uu.reshape(-1), vv.reshape(-1), torch.ones_like(uu).reshape(-1)], 0)
+    Minv = np.linalg.inv(P[:, :, :3])
+    C = -np.einsum("nij,nj->ni", Minv, P[:, :, 3])
+    S_all = torch.tensor((C - og) / sp, device=dev, dtype=f64)            # [n, 3]
+    A_all = torch.tensor(Minv / sp[None, :, None], device=dev, dtype=f64)  # [n, 3, 3]
+    chunk = max(1, int(2 ** 26 // max(1, Hd * Wd)))
     total = 0
-    for k in range(P.shape[0]):
-        M, p4 = P[k][:, :3], P[k][:, 3]
-        Minv = np.linalg.inv(M)
-        C = -Minv @ p4
-        s = torch.tensor((C - og) / sp, device=dev, dtype=f64)
-        A = torch.tensor(Minv / sp[:, None], device=dev, dtype=f64)
-        r = A @ pix
-        r = r / torch.linalg.norm(r, dim=0, keepdim=True)
-        amin = torch.full_like(r[0], -float("inf"))
-        amax = torch.full_like(r[0], float("inf"))
-        for a in range(3):
-            ra = r[a]
-            nz = ra != 0
-            inv = torch.where(nz, 1.0 / torch.where(nz, ra, torch.ones_like(ra)), torch.zeros_like(ra))
-            a0 = (lo - s[a]) * inv
-            a1 = (hi[a] - s[a]) * inv
-            mn = torch.where(nz, torch.minimum(a0, a1), torch.full_like(ra, -float("inf")))
-            mx = torch.where(nz, torch.maximum(a0, a1), torch.full_like(ra, float("inf")))
-            if a == 0:
-                amin = torch.where(nz, mn, torch.zeros_like(ra))
-            else:
-                amin = torch.maximum(amin, mn)
-            amax = torch.minimum(amax, mx)
-        a0 = amin + 0.5 * step
-        hit = a0 < amax
-        loop = torch.clamp(torch.ceil((amax - (a0 + step)) / step), min=0)
-        n = torch.where(hit, loop + 2, torch.zeros_like(loop))
-        total += int(n.sum().item())
+    inf = float("inf")
+    for k0 in range(0, P.shape[0], chunk):
+        A = A_all[k0:k0 + chunk]
+        s = S_all[k0:k0 + chunk, :, None]
+        r = A @ pix                                   # [c, 3, R]
+        r = r / torch.linalg.norm(r, dim=1, keepdim=True)
+        nz = r != 0
+        inv = 1.0 / torch.where(nz, r, torch.ones_like(r))
+        hi_t = torch.tensor(hi, device=dev, dtype=f64)[None, :, None]
+        a0 = (lo - s) * inv
+        a1 = (hi_t - s) * inv
+        mn = torch.where(nz, torch.minimum(a0, a1), torch.full_like(r, -inf))
+        mx = torch.where(nz, torch.maximum(a0, a1), torch.full_like(r, inf))
+        # the x slab overwrites the initial [0, inf); y and z intersect with it
+        amin = torch.where(nz[:, 0], mn[:, 0], torch.zeros_like(mn[:, 0]))
+        amin = torch.maximum(amin, torch.maximum(mn[:, 1], mn[:, 2]))
+        amax = torch.minimum(mx[:, 0], torch.minimum(mx[:, 1], mx[:, 2]))
+        first = amin + 0.5 * step
+        hit = first < amax
+        loop = torch.clamp(torch.ceil((amax - (first + step)) / step), min=0)
+        total += int(torch.where(hit, loop + 2, torch.zeros_like(loop)).sum().item())
+        del r, nz, inv, a0, a1, mn, mx
     return total
 
 

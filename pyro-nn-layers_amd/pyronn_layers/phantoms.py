@@ -171,7 +171,7 @@ def ellipse_line_integrals(e: Ellipses, src: np.ndarray, dirs: np.ndarray) -> np
 
 
 def rasterise_ellipsoids_torch(e: Ellipsoids, volume_shape, spacing, origin, device="cuda",
-                               dtype=None, slab: int = 16):
+                               dtype=None, slab: int = 128):
     """Same rasterisation as rasterise_ellipsoids, computed with torch on `device`
     (for full-size phantoms).  Accumulates in float64, returns float32."""
     import torch

@@ -81,8 +81,11 @@ def test_workspace_sizes_scale_with_storage():
     f32 = L.pyronn_cone_projection3d_workspace_size(1, 64, 64, 64, 10, 0)
     bf16 = L.pyronn_cone_projection3d_workspace_size(1, 64, 64, 64, 10, 1)
     assert f32 > bf16 > 0
+    # per view: (H_d + 2) x (W_d + 2) cells of 4 neighbours
     assert L.pyronn_cone_backprojection3d_workspace_size(2, 10, 8, 8, 0) == \
-        ((2 * 10 * 11 * 11 * 4 + 255) // 256) * 256
+        ((2 * 10 * 10 * 10 * 4 * 4 + 255) // 256) * 256
+    assert L.pyronn_cone_projection3d_workspace_size(1, 4, 4, 4, 2, 1) == \
+        ((1 * 2 * 12 * 4 + 255) // 256) * 256 + ((6 * 6 * 6 * 8 * 2 + 255) // 256) * 256
 
 
 def test_torch_ops_registered_and_cpu_tensors_rejected():
