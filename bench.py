@@ -57,6 +57,11 @@ def main(argv=None):
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 1 GiB device copy (reference for HBM byte counters)")
     ap.add_argument("--bf16-only", action="store_true", help="time only the BF16-storage run")
+    ap.add_argument("--no-serve", action="store_true", help="skip the FDK serving run")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--serve-only", action="store_true")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path.",
@@ -129,6 +134,9 @@ def main(argv=None):
         return float(np.mean(times_f)), float(np.mean(times_b)), sino
 
     # Timed region: K steps bracketed by synchronize on both sides (inside run()).
+    if args.serve_only:
+        print(json.dumps(serve_bench(args)), flush=True)
+        return 0
     if args.bf16_only:
         tf, tb, _ = run(torch.bfloat16)
         log(f"bf16 only: fwd {tf * 1e3:.1f} ms, bp {tb * 1e3:.1f} ms")
@@ -198,8 +206,103 @@ def main(argv=None):
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(vol, P, sp, og, cfg, args)
 
+    if not args.no_serve:
+        del vol
+        torch.cuda.empty_cache()
+        out["fdk_serving"] = serve_bench(args)
+
     print(json.dumps(out), flush=True)
     return 0
+
+
+def serve_bench(args):
+    """FDK serving: seeded Poisson arrivals of 40% fan 512^2 / 360 views, 40% cone
+    256^3 / 360 views (400x400 detector), 20% cone 512^3 / 720 views (1024x768);
+    each request carries a noisy sinogram in pinned host memory and goes through
+    cosine weighting, Ram-Lak filtering and backprojection; continuous batching
+    over `serve_slots` streams with HIP graphs, admission by an HBM budget.
+    Reported for FP32 and BF16 storage."""
+    import numpy as np
+    import torch
+
+    import pyronn_layers as pl
+    from pyronn_layers import geometry as geo
+    from pyronn_layers import phantoms as ph
+    from pyronn_layers.fdk import ConeFDK, FanFBP
+    from pyronn_layers.serving import GpuExecutor, Scheduler, latency_report, poisson_arrivals
+
+    pl.load()
+    dev = torch.device("cuda:0")
+    ops = torch.ops.pyronn_layers
+    geoms = {
+        "fan512": geo.Geometry2D(volume_shape=(512, 512), volume_spacing=(0.5, 0.5), detector_shape=725,
+                                 detector_spacing=1.0, n_views=360, angular_range=2 * math.pi, sid=1000.0,
+                                 sdd=1200.0),
+        "cone256": geo.ConeGeometry((256, 256, 256), (1.0, 1.0, 1.0), (400, 400), (1.0, 1.0), 360, 750.0, 1200.0),
+        "cone512": geo.ConeGeometry((512, 512, 512), (1.0, 1.0, 1.0), (768, 1024), (1.2, 1.2), 720, 750.0, 1200.0),
+    }
+    mix = {"fan512": 0.4, "cone256": 0.4, "cone512": 0.2}
+    rng = np.random.default_rng(1234)
+    clean = {}
+    log("serving: payloads")
+    for k, g in geoms.items():
+        if k.startswith("fan"):
+            e = ph.random_ellipses(11, 16, g.volume_shape, g.volume_spacing)
+            img = torch.tensor(ph.rasterise_ellipses(e, g.volume_shape, g.volume_spacing, g.volume_origin)[None],
+                               device=dev)
+            rays = torch.tensor(geo.fan_central_ray_vectors(g.n_views)[None], dtype=torch.float32, device=dev)
+            one = lambda v: torch.full((1, 1), v, dtype=torch.float32, device=dev)
+            sino = ops.fan_projection2d(img, [g.n_views, g.detector_shape],
+                                        torch.tensor(g.volume_origin[None], dtype=torch.float32, device=dev),
+                                        one(g.detector_origin),
+                                        torch.tensor(np.asarray(g.volume_spacing)[None], dtype=torch.float32,
+                                                     device=dev), one(g.detector_spacing), one(g.sid), one(g.sdd),
+                                        rays, 0.5)
+        else:
+            e = ph.random_ellipsoids(12, 20, g.volume_shape, g.volume_spacing,
+                                     axes_range_vox=(8.0, 100.0 * g.volume_shape[0] / 256))
+            vol = ph.rasterise_ellipsoids_torch(e, g.volume_shape, g.volume_spacing, g.volume_origin, device=dev)
+            sino = ops.cone_projection3d(vol[None], list(g.projection_shape),
+                                         torch.tensor(g.volume_origin[None], dtype=torch.float32, device=dev),
+                                         torch.tensor(np.asarray(g.volume_spacing)[None], dtype=torch.float32,
+                                                      device=dev),
+                                         torch.tensor(g.projection_matrices()[None], dtype=torch.float32,
+                                                      device=dev),
+                                         torch.full((1, 1), g.step_size, device=dev), torch.ones((1, 1), device=dev),
+                                         False)
+            del vol
+        clean[k] = sino
+    out = {"workload": "Poisson arrivals (seed 5, %.1f req/s, %d requests): 40%% fan 512^2/360 views, 40%% cone "
+                       "256^3/360 views 400x400, 20%% cone 512^3/720 views 1024x768; noisy sinograms "
+                       "(sigma = 0.01 max) from pinned host memory; latency = arrival -> volume in host memory"
+                       % (args.serve_rate, args.serve_requests),
+           "slots": args.serve_slots, "hip_graphs": True}
+    for name, storage in (("f32", torch.float32), ("bf16", torch.bfloat16)):
+        payloads = {}
+        for k, s_ in clean.items():
+            payloads[k] = []
+            for _ in range(2):
+                noisy = s_ + torch.randn(s_.shape, device=dev, generator=torch.Generator(device=dev).manual_seed(
+                    int(rng.integers(1 << 30)))) * (0.01 * float(s_.max()))
+                payloads[k].append(noisy.to(storage).cpu().pin_memory())
+        plans = {k: (FanFBP(g, storage=storage) if k.startswith("fan") else ConeFDK(g, storage=storage))
+                 for k, g in geoms.items()}
+        ex = GpuExecutor(plans, slots=args.serve_slots, use_graphs=True)
+        log(f"serving {name}: capturing graphs")
+        ex.warm()
+        reqs = poisson_arrivals(5, args.serve_rate, args.serve_requests, mix)
+        for r in reqs:
+            r.payload = payloads[r.kind][r.rid % 2]
+        log(f"serving {name}: {len(reqs)} requests")
+        res = Scheduler(ex, slots=args.serve_slots, hbm_budget=160 << 30).run(reqs, idle=lambda dt: time.sleep(
+            dt) if dt > 0 else None)
+        rep = latency_report(res)
+        log(f"serving {name}: {rep['req_per_s']:.2f} req/s p50 {rep['p50_ms']:.1f} ms p99 {rep['p99_ms']:.1f} ms")
+        out[name] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in rep.items()}
+        del ex, plans, payloads
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(vol_t, P, sp, og, cfg, args):

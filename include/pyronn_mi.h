@@ -170,6 +170,39 @@ int pyronn_fan_backprojection2d(const void *sinogram, int sinogram_dtype, int ba
                                 const float *central_ray_vectors, int H, int W, float *volume,
                                 pyronn_stream_t stream);
 
+/* --------------------------------------------------------------------------
+ * FDK / FBP pre-processing.  No counterpart in the reference: its FDK weights
+ * and filters live in the companion pyronn package (not part of
+ * PYRO-NN-Layers); the backprojectors above expect projections already
+ * weighted and filtered, with the FDK normalisation passed as
+ * projection_multiplier (cc/kernels/cone_backprojector_3D_CudaKernel.cu.cc:117).
+ * Formulas: Kak & Slaney, ch. 3 (Ram-Lak kernel, cosine pre-weighting).
+ * -------------------------------------------------------------------------- */
+
+/* T[j][c] = tau * h(c - j) * column_weights[c]: the zero-padded Ram-Lak filter
+ * of a row of det_w samples spaced tau (mm) as a Toeplitz matrix (device,
+ * det_w^2 floats), with an optional per-output-sample weight folded in
+ * (column_weights: device float[det_w] or NULL). */
+int pyronn_fdk_ramp_matrix(int det_w, double tau, const float *column_weights, float *T,
+                           pyronn_stream_t stream);
+
+/* weights[r][c] = sdd / sqrt(sdd^2 + ((c - u0) du)^2 + ((r - v0) dv)^2)
+ * (device, det_h * det_w floats; det_h = 1, dv = 0 for a fan-beam row). */
+int pyronn_fdk_cosine_weights(int det_h, int det_w, double du, double dv, double u0, double v0,
+                              double sdd, float *weights, pyronn_stream_t stream);
+
+/* Workspace bytes for pyronn_fdk_filter. */
+size_t pyronn_fdk_filter_workspace_size(int images, int det_h, int det_w);
+
+/* filtered[i][r][:] = (weights[r][:] .* sinogram[i][r][:]) x T for each of the
+ * `images` projection images (weights may be NULL: no weighting).  The row
+ * product runs as one rocBLAS SGEMM (a rocBLAS handle is created on the first
+ * call per thread and device); the output is stored as filtered_dtype. */
+int pyronn_fdk_filter(const void *sinogram, int sinogram_dtype, int images, int det_h, int det_w,
+                      const float *weights, const float *ramp_matrix, void *filtered,
+                      int filtered_dtype, void *workspace, size_t workspace_bytes,
+                      pyronn_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,7 +23,7 @@ OUT_DIR = os.path.join(PKG_ROOT, "pyronn_layers", "lib")
 HIP_LIB = os.path.join(OUT_DIR, "libpyronn_mi.so")
 OPS_LIB = os.path.join(OUT_DIR, "libpyronn_torch_ops.so")
 
-HIP_SOURCES = ["abi_common.cpp", "cone.hip", "planar.hip"]
+HIP_SOURCES = ["abi_common.cpp", "cone.hip", "planar.hip", "fdk.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
@@ -60,7 +60,8 @@ def build_hip(verbose=False) -> str:
                 print(err)
         objs.append(obj)
     tmp = HIP_LIB + ".tmp"
-    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, HIP_LIB)
     return HIP_LIB
 

@@ -1,0 +1,198 @@
+"""Filtered backprojection: FDK (cone, circular orbit) and FBP (fan / parallel).
+
+Not part of PYRO-NN-Layers itself: the reference's backprojectors expect
+weighted, filtered projections and take the FDK normalisation as
+`projection_multiplier` (cc/kernels/cone_backprojector_3D_CudaKernel.cu.cc:117);
+weighting and filtering lived in the companion pyronn package.  Here they are
+formulated after Kak & Slaney, ch. 3, for full (2 pi) orbits:
+
+  cone  f(x) = (pi / n) SID^2 sum_n Q_n(u_n, v_n) / w_n^2,  Q = cosine-weighted
+        projection filtered along detector rows with the Ram-Lak kernel at the
+        isocentre sample spacing tau = du SID / SDD; w_n is the depth along the
+        central ray, exactly the reference backprojector's 1/w^2 weight, so the
+        plan passes projection_multiplier = pi SID^2 / n.
+  fan   the same with one detector row.  The reference fan backprojector
+        (fan_backprojector_2D_CudaKernel.cu.cc:56-64) weights by 1/|x - s|^2 and
+        scales by SID SDD pi / n; |x - s| = depth / cos(gamma) with gamma fixed by
+        the detector position, so the filter output is multiplied by
+        (SDD^2 + s^2) / SDD^2 * SID / SDD to make the sum the textbook one.
+  par   f(x) = (pi / n) sum_n Q_n(x . theta_n) (the reference's pi / n scaling),
+        Q = R filtered at tau = detector spacing.
+
+Plans keep all geometry and filter state resident on the device (projection
+matrices, cosine weights, the Toeplitz ramp matrix, workspaces), so one
+reconstruction is: one weighting kernel + one rocBLAS GEMM (+ a storage
+conversion for BF16/FP16) + one backprojection, all on the current stream and
+capturable in a HIP graph (see `capture`).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import geometry as geo
+from . import load as _load_ops
+from . import native as nat
+
+_DT = {torch.float32: nat.F32, torch.bfloat16: nat.BF16, torch.float16: nat.F16}
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class _Plan:
+    storage: torch.dtype
+    device: torch.device
+
+    def _filter(self, sino: torch.Tensor, images: int, det_h: int, det_w: int) -> torch.Tensor:
+        sino = sino.contiguous()
+        code = _DT.get(sino.dtype)
+        if code is None:
+            raise TypeError(f"sinogram dtype {sino.dtype} not supported")
+        out = torch.empty(sino.shape, dtype=self.storage, device=self.device)
+        # Per-call workspace from torch's stream-aware caching allocator (graph-pool
+        # memory under capture), so concurrent calls on different streams never share it.
+        ws_bytes = nat.lib().pyronn_fdk_filter_workspace_size(images, det_h, det_w)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=self.device)
+        w = _p(self.weights) if self.weights is not None else None
+        nat.call("pyronn_fdk_filter", _p(sino), code, images, det_h, det_w, w, _p(self.ramp), _p(out),
+                 _DT[self.storage], _p(ws), ws.numel(), _stream())
+        return out
+
+    def capture(self, example: torch.Tensor, warmup: int = 2):
+        """Capture filter + backprojection for inputs shaped like `example` in a
+        HIP graph.  Returns (graph, static_input, static_output): copy a request
+        into static_input, replay the graph, read static_output."""
+        static_in = example.clone()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for _ in range(warmup):
+                self(static_in)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            static_out = self(static_in)
+        return g, static_in, static_out
+
+
+class ConeFDK(_Plan):
+    """FDK for a full circular cone-beam orbit described by a ConeGeometry."""
+
+    def __init__(self, g: geo.ConeGeometry, device="cuda", storage: torch.dtype = torch.float32):
+        _load_ops()
+        self.g = g
+        self.device = torch.device(device)
+        self.storage = storage
+        n = g.n_views
+        Hd, Wd = g.detector_shape
+        dv, du = g.detector_spacing
+        dev = self.device
+        self.P = torch.tensor(g.projection_matrices()[None], dtype=torch.float32, device=dev)
+        self.vo = torch.tensor(np.asarray(g.volume_origin)[None], dtype=torch.float32, device=dev)
+        self.vs = torch.tensor(np.asarray(g.volume_spacing)[None], dtype=torch.float32, device=dev)
+        self.step = torch.full((1, 1), g.step_size, dtype=torch.float32, device=dev)
+        dbeta = g.angular_range / n
+        self.mult_value = 0.5 * dbeta * g.sid ** 2
+        self.mult = torch.full((1, 1), self.mult_value, dtype=torch.float32, device=dev)
+        self.weights = torch.empty((Hd, Wd), dtype=torch.float32, device=dev)
+        nat.call("pyronn_fdk_cosine_weights", Hd, Wd, du, dv, (Wd - 1) / 2.0, (Hd - 1) / 2.0, g.sdd,
+                 _p(self.weights), _stream())
+        self.tau = du * g.sid / g.sdd
+        self.ramp = torch.empty((Wd, Wd), dtype=torch.float32, device=dev)
+        nat.call("pyronn_fdk_ramp_matrix", Wd, self.tau, None, _p(self.ramp), _stream())
+
+    def filter(self, sino: torch.Tensor) -> torch.Tensor:
+        B, n, Hd, Wd = sino.shape
+        return self._filter(sino, B * n, Hd, Wd)
+
+    def __call__(self, sino: torch.Tensor) -> torch.Tensor:
+        """sino [B, n, H_d, W_d] (float32 / bfloat16 / float16) -> volume [B, D, H, W] float32."""
+        if sino.dim() == 3:
+            sino = sino[None]
+        q = self.filter(sino)
+        return torch.ops.pyronn_layers.cone_backprojection3d(q, list(self.g.volume_shape), self.vo, self.vs,
+                                                             self.P, self.step, self.mult, False)
+
+
+class FanFBP(_Plan):
+    """FBP for a full fan-beam orbit (equispaced flat detector) given a Geometry2D with sid/sdd."""
+
+    def __init__(self, g: geo.Geometry2D, device="cuda", storage: torch.dtype = torch.float32):
+        _load_ops()
+        self.g = g
+        self.device = torch.device(device)
+        self.storage = storage
+        dev = self.device
+        det, ds = g.detector_shape, g.detector_spacing
+        self.rays = torch.tensor(geo.fan_central_ray_vectors(g.n_views, g.angular_range)[None],
+                                 dtype=torch.float32, device=dev)
+        self.vo = torch.tensor(np.asarray(g.volume_origin)[None], dtype=torch.float32, device=dev)
+        self.vs = torch.tensor(np.asarray(g.volume_spacing)[None], dtype=torch.float32, device=dev)
+        self.do = torch.full((1, 1), g.detector_origin, dtype=torch.float32, device=dev)
+        self.ds = torch.full((1, 1), ds, dtype=torch.float32, device=dev)
+        self.sid = torch.full((1, 1), g.sid, dtype=torch.float32, device=dev)
+        self.sdd = torch.full((1, 1), g.sdd, dtype=torch.float32, device=dev)
+        u0 = -g.detector_origin / ds
+        self.weights = torch.empty((1, det), dtype=torch.float32, device=dev)
+        nat.call("pyronn_fdk_cosine_weights", 1, det, ds, 0.0, u0, 0.0, g.sdd, _p(self.weights), _stream())
+        s = g.detector_origin + ds * np.arange(det)
+        # 1/cos^2(gamma) undoes the reference's Euclidean 1/|x-s|^2 weight, SID/SDD its SDD factor;
+        # full orbit: (1/2) dbeta = pi / n, which the reference already applies.
+        post = ((g.sdd ** 2 + s ** 2) / g.sdd ** 2) * (g.sid / g.sdd) * (g.angular_range / (2 * math.pi))
+        colw = torch.tensor(post, dtype=torch.float32, device=dev)
+        self.tau = ds * g.sid / g.sdd
+        self.ramp = torch.empty((det, det), dtype=torch.float32, device=dev)
+        nat.call("pyronn_fdk_ramp_matrix", det, self.tau, _p(colw), _p(self.ramp), _stream())
+        torch.cuda.current_stream().synchronize()
+
+    def __call__(self, sino: torch.Tensor) -> torch.Tensor:
+        """sino [B, n, det] -> image [B, H, W] float32."""
+        if sino.dim() == 2:
+            sino = sino[None]
+        B, n, det = sino.shape
+        q = self._filter(sino, B * n, 1, det)
+        return torch.ops.pyronn_layers.fan_backprojection2d(q, list(self.g.volume_shape), self.vo, self.do,
+                                                            self.vs, self.ds, self.sid, self.sdd, self.rays)
+
+
+class ParallelFBP(_Plan):
+    """FBP for parallel-beam views over pi (Geometry2D)."""
+
+    def __init__(self, g: geo.Geometry2D, device="cuda", storage: torch.dtype = torch.float32):
+        _load_ops()
+        self.g = g
+        self.device = torch.device(device)
+        self.storage = storage
+        dev = self.device
+        det, ds = g.detector_shape, g.detector_spacing
+        self.rays = torch.tensor(geo.parallel_ray_vectors(g.n_views, g.angular_range)[None], dtype=torch.float32,
+                                 device=dev)
+        self.vo = torch.tensor(np.asarray(g.volume_origin)[None], dtype=torch.float32, device=dev)
+        self.vs = torch.tensor(np.asarray(g.volume_spacing)[None], dtype=torch.float32, device=dev)
+        self.do = torch.full((1, 1), g.detector_origin, dtype=torch.float32, device=dev)
+        self.ds = torch.full((1, 1), ds, dtype=torch.float32, device=dev)
+        self.weights = None
+        self.tau = ds
+        # The reference's pi / n scaling is the FBP weight for views over pi and,
+        # since every direction then appears twice, also for views over 2 pi.
+        self.ramp = torch.empty((det, det), dtype=torch.float32, device=dev)
+        nat.call("pyronn_fdk_ramp_matrix", det, self.tau, None, _p(self.ramp), _stream())
+        torch.cuda.current_stream().synchronize()
+
+    def __call__(self, sino: torch.Tensor) -> torch.Tensor:
+        if sino.dim() == 2:
+            sino = sino[None]
+        B, n, det = sino.shape
+        q = self._filter(sino, B * n, 1, det)
+        return torch.ops.pyronn_layers.parallel_backprojection2d(q, list(self.g.volume_shape), self.vo, self.do,
+                                                                 self.vs, self.ds, self.rays)
