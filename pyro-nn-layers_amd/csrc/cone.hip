@@ -68,36 +68,67 @@ __global__ void cone_setup_kernel(const float *__restrict__ P, const float *__re
 // with zeros outside the volume.  One sample then costs one 16-byte load (bf16 /
 // f16 storage) or two (f32) instead of eight 4-byte gathers, and the zero border
 // makes the reference's per-neighbour bounds tests unnecessary.
+// Cells are stored z-fastest: index ((x+1) (H+2) + (y+1)) (D+2) + (z+1).  For an
+// orbit about the z axis, z is transverse to every ray, so the 64 rays of a wave
+// (8 detector rows ~ 8 z positions) fall into few cache lines for every view
+// direction.  Other trajectories are handled identically, only less cache-friendly.
+// The build goes through a z-fastest transpose of the volume so that both of its
+// passes read and write contiguously.
 // =====================================================================================
 template <typename TS>
-__device__ __forceinline__ TS vox(const TS *__restrict__ v, int x, int y, int z, int W, int H,
-                                  int D) {
-  if (x < 0 || x >= W || y < 0 || y >= H || z < 0 || z >= D) return zero_of<TS>();
-  return v[((size_t)z * H + y) * W + x];
+__global__ void __launch_bounds__(256)
+transpose_zx_kernel(const TS *__restrict__ vol, TS *__restrict__ vt, int W, int H, int D) {
+  // vol [b][z][y][x] -> vt [b][x][y][z]; one 32 x 32 (z, x) tile of one y plane per block
+  __shared__ TS tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const int x0 = blockIdx.x * 32, z0 = (blockIdx.y % ((D + 31) / 32)) * 32;
+  const int y = blockIdx.y / ((D + 31) / 32);
+  const int b = blockIdx.z;
+  const size_t plane = (size_t)W * H * D;
+  const TS *src = vol + b * plane;
+  TS *dst = vt + b * plane;
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const int z = z0 + ty + k, x = x0 + tx;
+    if (z < D && x < W) tile[ty + k][tx] = src[((size_t)z * H + y) * W + x];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const int x = x0 + ty + k, z = z0 + tx;
+    if (z < D && x < W) dst[((size_t)x * H + y) * D + z] = tile[tx][ty + k];
+  }
 }
 
 template <typename TS>
-__global__ void build_cells_kernel(const TS *__restrict__ vol, TS *__restrict__ cells, int W,
+__device__ __forceinline__ TS voxt(const TS *__restrict__ vt, int x, int y, int z, int W, int H,
+                                   int D) {
+  if (x < 0 || x >= W || y < 0 || y >= H || z < 0 || z >= D) return zero_of<TS>();
+  return vt[((size_t)x * H + y) * D + z];
+}
+
+template <typename TS>
+__global__ void build_cells_kernel(const TS *__restrict__ vt, TS *__restrict__ cells, int W,
                                    int H, int D, long long total) {
   const int Wc = W + 2, Hc = H + 2, Dc = D + 2;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
     long long r = t;
-    const int cx = (int)(r % Wc); r /= Wc;
+    const int cz = (int)(r % Dc); r /= Dc;
     const int cy = (int)(r % Hc); r /= Hc;
-    const int cz = (int)(r % Dc);
-    const long long b = r / Dc;
-    const TS *v = vol + (size_t)b * W * H * D;
+    const int cx = (int)(r % Wc);
+    const long long b = r / Wc;
+    const TS *v = vt + (size_t)b * W * H * D;
     const int x = cx - 1, y = cy - 1, z = cz - 1;
     TS c[8];
-    c[0] = vox(v, x, y, z, W, H, D);
-    c[1] = vox(v, x + 1, y, z, W, H, D);
-    c[2] = vox(v, x, y + 1, z, W, H, D);
-    c[3] = vox(v, x + 1, y + 1, z, W, H, D);
-    c[4] = vox(v, x, y, z + 1, W, H, D);
-    c[5] = vox(v, x + 1, y, z + 1, W, H, D);
-    c[6] = vox(v, x, y + 1, z + 1, W, H, D);
-    c[7] = vox(v, x + 1, y + 1, z + 1, W, H, D);
+    c[0] = voxt(v, x, y, z, W, H, D);
+    c[1] = voxt(v, x + 1, y, z, W, H, D);
+    c[2] = voxt(v, x, y + 1, z, W, H, D);
+    c[3] = voxt(v, x + 1, y + 1, z, W, H, D);
+    c[4] = voxt(v, x, y, z + 1, W, H, D);
+    c[5] = voxt(v, x + 1, y, z + 1, W, H, D);
+    c[6] = voxt(v, x, y + 1, z + 1, W, H, D);
+    c[7] = voxt(v, x + 1, y + 1, z + 1, W, H, D);
     TS *o = cells + (size_t)t * 8;
 #pragma unroll
     for (int q = 0; q < 8; ++q) o[q] = c[q];
@@ -118,8 +149,9 @@ __device__ __forceinline__ void unpack2(uint32_t w, _Float16, float &a, float &b
 // Trilinear sample at (px, py, pz), lerp z -> y -> x exactly as
 // cone_projector_3D_CudaKernel.cu.cc:74-82.  The coordinate is clamped into the
 // zero border first (identical results for every finite coordinate).
+// Dc = D + 2, HDc = (H + 2) * (D + 2): strides of the z-fastest cell array.
 template <typename TS>
-__device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int Wc, int HWc,
+__device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int Dc, int HDc,
                                               float px, float py, float pz, float fW, float fH,
                                               float fD) {
   px = clamp_apron(px, fW);
@@ -127,7 +159,7 @@ __device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int 
   pz = clamp_apron(pz, fD);
   const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
   const float xd = px - fx, yd = py - fy, zd = pz - fz;
-  const unsigned cell = (unsigned)(((int)fz + 1) * HWc + ((int)fy + 1) * Wc + ((int)fx + 1));
+  const unsigned cell = (unsigned)(((int)fx + 1) * HDc + ((int)fy + 1) * Dc + ((int)fz + 1));
   float p000, p001, p010, p011, p100, p101, p110, p111;
   if constexpr (sizeof(TS) == 4) {
     const float4 *c4 = reinterpret_cast<const float4 *>(cells) + (size_t)cell * 2;
@@ -151,14 +183,14 @@ __device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int 
 }
 
 template <typename TS, bool HW>
-__device__ __forceinline__ float cone_sample(const TS *__restrict__ c, int Wc, int HWc, float px,
+__device__ __forceinline__ float cone_sample(const TS *__restrict__ c, int Dc, int HDc, float px,
                                              float py, float pz, float fW, float fH, float fD) {
   if (HW) {  // tex3D(p + 0.5): the sampler subtracts the half texel again
     px = (px + 0.5f) - 0.5f;
     py = (py + 0.5f) - 0.5f;
     pz = (pz + 0.5f) - 0.5f;
   }
-  return sample_cells(c, Wc, HWc, px, py, pz, fW, fH, fD);
+  return sample_cells(c, Dc, HDc, px, py, pz, fW, fH, fD);
 }
 
 // =====================================================================================
@@ -203,7 +235,7 @@ cone_fwd_kernel(const TS *__restrict__ cells, float *__restrict__ sino,
   const float step = step_ptr[b];
   const float spx = spacing[b * 3 + 2], spy = spacing[b * 3 + 1], spz = spacing[b * 3 + 0];
   const TS *v = cells + (size_t)b * cell_stride * 8;
-  const int Wp = W + 2, HWp = (H + 2) * Wp;
+  const int Wp = D + 2, HWp = (H + 2) * Wp;  // z-fastest cell strides
   const float fW = (float)W, fH = (float)H, fD = (float)D;
   float *out = sino + (vb * Hd + row) * (size_t)Wd + col;
 
@@ -415,13 +447,15 @@ static long long fwd_blocks(int n, int Hd, int Wd) {
 }
 
 template <typename TS>
-static void launch_cells(const void *vol, void *cells, int B, int D, int H, int W,
+static void launch_cells(const void *vol, void *vt, void *cells, int B, int D, int H, int W,
                          hipStream_t stream) {
+  dim3 tgrid((W + 31) / 32, ((D + 31) / 32) * H, B);
+  transpose_zx_kernel<TS><<<tgrid, dim3(256), 0, stream>>>((const TS *)vol, (TS *)vt, W, H, D);
   const long long total = (long long)B * volume_cells(D, H, W);
   long long blocks = (total + 255) / 256;
   if (blocks > 65536 * 4) blocks = 65536 * 4;
   build_cells_kernel<TS><<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(
-      (const TS *)vol, (TS *)cells, W, H, D, total);
+      (const TS *)vt, (TS *)cells, W, H, D, total);
 }
 
 template <typename TS>
@@ -495,7 +529,8 @@ extern "C" size_t pyronn_cone_projection3d_workspace_size(int batch, int D, int 
                                                           int n_views, int volume_dtype) {
   if (batch <= 0 || D <= 0 || H <= 0 || W <= 0 || n_views <= 0 || !dtype_ok(volume_dtype)) return 0;
   return fwd_geom_bytes(batch, n_views) +
-         align_up((size_t)batch * volume_cells(D, H, W) * 8 * dtype_bytes(volume_dtype), 256);
+         align_up((size_t)batch * volume_cells(D, H, W) * 8 * dtype_bytes(volume_dtype), 256) +
+         align_up((size_t)batch * D * H * W * dtype_bytes(volume_dtype), 256);
 }
 
 extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, int batch, int D,
@@ -520,6 +555,10 @@ extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, in
     set_error("cone_projection3d: volume %dx%dx%d exceeds the 2^31-cell limit", D, H, W);
     return PYRONN_ERR_TOO_LARGE;
   }
+  if ((long long)((D + 31) / 32) * H > 65535) {
+    set_error("cone_projection3d: volume %dx%dx%d too large for the transpose grid", D, H, W);
+    return PYRONN_ERR_TOO_LARGE;
+  }
   if (fwd_blocks(n_views, det_h, det_w) >= (long long)INT32_MAX) {
     set_error("cone_projection3d: %d views of %dx%d pixels exceed one launch", n_views, det_h, det_w);
     return PYRONN_ERR_TOO_LARGE;
@@ -534,21 +573,23 @@ extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, in
   float *inv_ar = (float *)ws;
   float *srcp = inv_ar + (size_t)batch * n_views * 9;
   void *cells = ws + fwd_geom_bytes(batch, n_views);
+  void *vt = (char *)cells +
+             align_up((size_t)batch * volume_cells(D, H, W) * 8 * dtype_bytes(volume_dtype), 256);
 
   int rc = pyronn_cone_geometry_setup(projection_matrices, volume_origin, volume_spacing, batch,
                                       n_views, inv_ar, srcp, stream);
   if (rc != PYRONN_OK) return rc;
   const bool hw = hardware_interp != 0;
   if (volume_dtype == PYRONN_F32) {
-    launch_cells<float>(volume, cells, batch, D, H, W, s);
+    launch_cells<float>(volume, vt, cells, batch, D, H, W, s);
     launch_fwd<float>(hw, cells, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H, W,
                       n_views, det_h, det_w, s);
   } else if (volume_dtype == PYRONN_BF16) {
-    launch_cells<bf16s>(volume, cells, batch, D, H, W, s);
+    launch_cells<bf16s>(volume, vt, cells, batch, D, H, W, s);
     launch_fwd<bf16s>(hw, cells, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H, W,
                       n_views, det_h, det_w, s);
   } else {
-    launch_cells<_Float16>(volume, cells, batch, D, H, W, s);
+    launch_cells<_Float16>(volume, vt, cells, batch, D, H, W, s);
     launch_fwd<_Float16>(hw, cells, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H,
                          W, n_views, det_h, det_w, s);
   }
