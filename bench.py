@@ -152,6 +152,13 @@ def main(argv=None):
     B_bp = U_bp * 4 * 4 + V * 4
     dominant = "cone_fwd_kernel" if tf >= tb else "cone_bp_kernel"
     B_dom, t_dom = (B_fwd, tf) if tf >= tb else (B_bp, tb)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "cone_m1_f32_traffic.json")
+    if dominant == "cone_fwd_kernel" and os.path.exists(tpath):
+        # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
+        # scaled by the reading of a 1 GiB copy (see that file's source summary)
+        with open(tpath) as f:
+            traffic = json.load(f)["traffic_bytes"]
     out = {
         "metric": METRIC,
         "value": round(gs, 3),
@@ -174,7 +181,8 @@ def main(argv=None):
         "roofline": {
             "bound": "hbm", "kernel": dominant,
             "achieved": round(B_dom / t_dom / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": round(B_dom / t_dom / HBM_PEAK, 4), "traffic": None,
+            "frac": round(B_dom / t_dom / HBM_PEAK, 4), "traffic": traffic,
+            "traffic_source": "profiles/cone_m1_f32_traffic.json (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
         },
         "fwd_back": {
             "t_fwd_ms": round(tf * 1e3, 2), "t_bp_ms": round(tb * 1e3, 2),
