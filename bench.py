@@ -62,6 +62,8 @@ def main(argv=None):
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the uneven-batch run")
+    ap.add_argument("--batched-only", action="store_true")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path.",
@@ -136,6 +138,9 @@ def main(argv=None):
     # Timed region: K steps bracketed by synchronize on both sides (inside run()).
     if args.serve_only:
         print(json.dumps(serve_bench(args)), flush=True)
+        return 0
+    if args.batched_only:
+        print(json.dumps(batched_bench(args)), flush=True)
         return 0
     if args.bf16_only:
         tf, tb, _ = run(torch.bfloat16)
@@ -214,10 +219,13 @@ def main(argv=None):
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(vol, P, sp, og, cfg, args)
 
+    del vol
+    torch.cuda.empty_cache()
     if not args.no_serve:
-        del vol
-        torch.cuda.empty_cache()
         out["fdk_serving"] = serve_bench(args)
+    if not args.no_batched:
+        torch.cuda.empty_cache()
+        out["uneven_batch"] = batched_bench(args)
 
     print(json.dumps(out), flush=True)
     return 0
@@ -309,6 +317,93 @@ def serve_bench(args):
         out[name] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in rep.items()}
         del ex, plans, payloads
         torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    return out
+
+
+def batched_bench(args):
+    """Eight uneven cone problems (128^3 ... 512^3, 360-720 views, detectors 256^2 ...
+    1024x768, SID 600-1000 mm, each with its own seeded 20-ellipsoid phantom): forward
+    and backprojection each in ONE persistent launch with autotuned grid parameters,
+    FP32 and BF16 storage; versus the same problems launched one after another; error
+    of the GPU forward against the FP64 oracle on views of the smallest problem."""
+    import numpy as np
+    import torch
+
+    import oracle as orc
+    import pyronn_layers as pl
+    from pyronn_layers import batched, metrics
+    from pyronn_layers import geometry as geo
+    from pyronn_layers import phantoms as ph
+    from pyronn_layers.autotune import Autotuner
+
+    pl.load()
+    dev = torch.device("cuda:0")
+    sizes = [128, 192, 256, 256, 320, 384, 448, 512]
+    views = [360, 400, 450, 500, 560, 620, 680, 720]
+    sids = np.linspace(600.0, 1000.0, len(sizes))
+    geoms, vols = [], []
+    log("uneven batch: phantoms")
+    for k, (N, n, sid) in enumerate(zip(sizes, views, sids)):
+        det = (max(256, N * 3 // 2), 2 * N)
+        g = geo.ConeGeometry((N, N, N), (1.0, 1.0, 1.0), det, (1.2, 1.2), n, float(sid), float(sid) * 1.6)
+        e = ph.random_ellipsoids(100 + k, 20, g.volume_shape, g.volume_spacing,
+                                 axes_range_vox=(8.0 * N / 256, 100.0 * N / 256))
+        geoms.append(g)
+        vols.append(ph.rasterise_ellipsoids_torch(e, g.volume_shape, g.volume_spacing, g.volume_origin, device=dev))
+    S = sum(metrics.cone_forward_sample_count(g.projection_matrices(), g.volume_shape, g.volume_spacing,
+                                              g.volume_origin, g.detector_shape, g.step_size, device=dev)
+            for g in geoms)
+    U = sum(metrics.cone_backprojection_update_count(g.volume_shape, g.n_views) for g in geoms)
+    tuner = Autotuner(cache_path=None, reps=1)
+
+    def timed(fn):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        r = fn()
+        b.record()
+        torch.cuda.synchronize()
+        return r, a.elapsed_time(b) / 1e3
+
+    out = {"workload": "8 cone problems: " + ", ".join(f"{g.volume_shape[0]}^3/{g.n_views}v/{g.detector_shape[1]}x"
+                                                      f"{g.detector_shape[0]}" for g in geoms),
+           "S_fwd": S, "U_bp": U}
+    for name, storage in (("f32", torch.float32), ("bf16", torch.bfloat16)):
+        vin = [v.to(storage) for v in vols]
+        log(f"uneven batch {name}: tuning")
+        tf = tuner.forward(vin, geoms)
+        sinos, t_fwd = timed(lambda: batched.cone_projection3d_batched(vin, geoms,
+                                                                       blocks_per_cu=tf["blocks_per_cu"]))
+        sin = [s_.to(storage) for s_ in sinos]
+        tb = tuner.backward(sin, geoms)
+        _, t_bp = timed(lambda: batched.cone_backprojection3d_batched(sin, geoms, z_per_lane=tb["z_per_lane"],
+                                                                      blocks_per_cu=tb["blocks_per_cu"]))
+        # the same problems launched one after another through the single-problem ops
+        def seq_fwd():
+            for v, g in zip(vin, geoms):
+                batched.cone_projection3d_batched([v], [g], blocks_per_cu=tf["blocks_per_cu"])
+        def seq_bp():
+            for s_, g in zip(sin, geoms):
+                batched.cone_backprojection3d_batched([s_], [g], z_per_lane=tb["z_per_lane"])
+        _, ts_f = timed(seq_fwd)
+        _, ts_b = timed(seq_bp)
+        # error against the FP64 oracle: two views of the 128^3 problem, forward
+        g0 = geoms[0]
+        inv, src, inv64, src64, _ = orc.cone_setup(g0.projection_matrices().astype(np.float32),
+                                                   np.float32(g0.volume_spacing), g0.volume_origin.astype(np.float32))
+        r64, _ = orc.cone_forward_f64(vols[0].double().cpu().numpy(), inv64, src64, g0.detector_shape,
+                                      np.asarray(g0.volume_spacing), g0.step_size, views=(0, 2))
+        got = sinos[0][:2].double().cpu().numpy()
+        err = float(np.linalg.norm(got - r64) / np.linalg.norm(r64))
+        log(f"uneven batch {name}: fwd {t_fwd * 1e3:.1f} ms bp {t_bp * 1e3:.1f} ms (sequential {ts_f * 1e3:.1f} / "
+            f"{ts_b * 1e3:.1f} ms), rel L2 vs FP64 {err:.2e}")
+        out[name] = {"gsamples_s": round((S + U) / (t_fwd + t_bp) / 1e9, 2), "t_fwd_ms": round(t_fwd * 1e3, 2),
+                     "t_bp_ms": round(t_bp * 1e3, 2), "sequential_t_fwd_ms": round(ts_f * 1e3, 2),
+                     "sequential_t_bp_ms": round(ts_b * 1e3, 2), "tuned_fwd": tf["blocks_per_cu"],
+                     "tuned_bp": [tb["z_per_lane"], tb["blocks_per_cu"]],
+                     "rel_l2_vs_fp64_fwd_views_0_1_of_128^3": err}
+        del sinos, sin, vin
         torch.cuda.empty_cache()
     return out
 

@@ -51,6 +51,9 @@ enum pyronn_status {
 
 enum pyronn_dtype { PYRONN_F32 = 0, PYRONN_BF16 = 1, PYRONN_F16 = 2 };
 
+/* Largest number of problems in one batched (persistent) launch. */
+#define PYRONN_MAX_BATCHED_PROBLEMS 16
+
 /* Message for the last failing call on this thread ("" if none). */
 const char *pyronn_last_error(void);
 
@@ -169,6 +172,41 @@ int pyronn_fan_backprojection2d(const void *sinogram, int sinogram_dtype, int ba
                                 const float *source_2_detector_distance,
                                 const float *central_ray_vectors, int H, int W, float *volume,
                                 pyronn_stream_t stream);
+
+/* --------------------------------------------------------------------------
+ * Batched cone operators over problems of different shapes and geometries.
+ * No single counterpart in the reference: ConeProjection3DOp::Compute
+ * (cc/ops/cone_projector_3D_OPKernel.cc:234-247) loops over batch items of one
+ * shape and launches once per item.  Here up to PYRONN_MAX_BATCHED_PROBLEMS
+ * volumes / sinograms of any shapes run in ONE persistent launch (plus small
+ * per-problem layout passes); every output equals the single-problem call bit
+ * for bit.  All problems share the storage dtype and hardware_interp.
+ * -------------------------------------------------------------------------- */
+typedef struct pyronn_cone_problem {
+  const void *input;                  /* forward: volume [D][H][W]; backprojection: sinogram [n][H_d][W_d] */
+  float *output;                      /* forward: sinogram [n][H_d][W_d]; backprojection: volume [D][H][W] */
+  const float *volume_origin;         /* [3] (z,y,x) */
+  const float *volume_spacing;        /* [3] (z,y,x) */
+  const float *projection_matrices;   /* [n][3][4] */
+  const float *step_size;             /* [1]  (read by the forward) */
+  const float *projection_multiplier; /* [1]  (read by the backprojection) */
+  int D, H, W, n_views, det_h, det_w;
+} pyronn_cone_problem;
+
+size_t pyronn_cone_projection3d_batched_workspace_size(const pyronn_cone_problem *problems, int count,
+                                                       int dtype);
+/* blocks_per_cu: persistent workgroups per CU (<= 0: 8), a tuning parameter. */
+int pyronn_cone_projection3d_batched(const pyronn_cone_problem *problems, int count, int dtype,
+                                     int hardware_interp, int blocks_per_cu, void *workspace,
+                                     size_t workspace_bytes, pyronn_stream_t stream);
+
+size_t pyronn_cone_backprojection3d_batched_workspace_size(const pyronn_cone_problem *problems,
+                                                           int count, int dtype);
+/* z_per_lane: voxels along z per lane, 4 or 8 (<= 0: 8 for f32, 4 for 16-bit storage). */
+int pyronn_cone_backprojection3d_batched(const pyronn_cone_problem *problems, int count, int dtype,
+                                         int hardware_interp, int z_per_lane, int blocks_per_cu,
+                                         void *workspace, size_t workspace_bytes,
+                                         pyronn_stream_t stream);
 
 /* --------------------------------------------------------------------------
  * FDK / FBP pre-processing.  No counterpart in the reference: its FDK weights

@@ -207,38 +207,43 @@ __device__ __forceinline__ float cone_sample(const TS *__restrict__ c, int Dc, i
 // Ray setup: cone_projector_3D_CudaKernel.cu.cc:197-221; march: :87-181
 // (hardware_interp: cone_projector_3D_CudaKernel_hardware_interp.cu.cc:43-142).
 // =====================================================================================
+// One forward problem (one volume and its views) as the kernels see it.
+struct FwdProblem {
+  const void *cells;     // z-fastest neighbourhood cells of the volume
+  float *sino;           // [n][H_d][W_d]
+  const float *inv_ar;   // [n][9]
+  const float *src;      // [n][3]
+  const float *step;     // [1]
+  const float *spacing;  // [3] (z, y, x)
+  int W, H, D, n, Hd, Wd, tiles_x, tiles_y;
+  long long begin;       // first work item of the problem in a batched launch (multiple of 8)
+};
+
 template <typename TS, bool HW>
-__global__ void __launch_bounds__(256)
-cone_fwd_kernel(const TS *__restrict__ cells, float *__restrict__ sino,
-                const float *__restrict__ inv_ar, const float *__restrict__ src,
-                const float *__restrict__ step_ptr, const float *__restrict__ spacing, int W,
-                int H, int D, int n, int Hd, int Wd, int tiles_x, int tiles_y,
-                long long cell_stride) {
+__device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int xcd = (int)(blockIdx.x % kXcds);
-  const long long q = blockIdx.x / kXcds;
-  const long long per_row = (long long)n * tiles_x;
-  const int band = (int)(q / per_row);
+  const int xcd = (int)(q % kXcds);
+  const long long qq = q / kXcds;
+  const long long per_row = (long long)P.n * P.tiles_x;
+  const int band = (int)(qq / per_row);
   const int ty = band * kXcds + xcd;
-  if (ty >= tiles_y) return;
-  const long long w_ = q - (long long)band * per_row;
-  const int view = (int)(w_ / tiles_x);
-  const int tx = (int)(w_ - (long long)view * tiles_x);
+  if (ty >= P.tiles_y) return;
+  const long long w_ = qq - (long long)band * per_row;
+  const int view = (int)(w_ / P.tiles_x);
+  const int tx = (int)(w_ - (long long)view * P.tiles_x);
   const int col = tx * 16 + (wave & 1) * 8 + (lane & 7);
   const int row = ty * 16 + (wave >> 1) * 8 + (lane >> 3);
-  const int b = blockIdx.z;
+  const int W = P.W, H = P.H, D = P.D, Hd = P.Hd, Wd = P.Wd;
   if (col >= Wd || row >= Hd) return;
 
-  const size_t vb = (size_t)b * n + view;
-  const float *m = inv_ar + vb * 9;
-  const float s0 = src[vb * 3 + 0], s1 = src[vb * 3 + 1], s2 = src[vb * 3 + 2];
-  const float step = step_ptr[b];
-  const float spx = spacing[b * 3 + 2], spy = spacing[b * 3 + 1], spz = spacing[b * 3 + 0];
-  const TS *v = cells + (size_t)b * cell_stride * 8;
+  const float *m = P.inv_ar + (size_t)view * 9;
+  const float s0 = P.src[view * 3 + 0], s1 = P.src[view * 3 + 1], s2 = P.src[view * 3 + 2];
+  const float step = P.step[0];
+  const float spx = P.spacing[2], spy = P.spacing[1], spz = P.spacing[0];
+  const TS *v = static_cast<const TS *>(P.cells);
   const int Wp = D + 2, HWp = (H + 2) * Wp;  // z-fastest cell strides
   const float fW = (float)W, fH = (float)H, fD = (float)D;
-  float *out = sino + (vb * Hd + row) * (size_t)Wd + col;
-
+  float *out = P.sino + ((size_t)view * Hd + row) * (size_t)Wd + col;
   const float u = (float)col, w = (float)row;
   float rx = fmaf(u, m[0], fmaf(w, m[1], m[2]));
   float ry = fmaf(u, m[3], fmaf(w, m[4], m[5]));
@@ -307,6 +312,64 @@ cone_fwd_kernel(const TS *__restrict__ cells, float *__restrict__ sino,
   }
   const float a = rx * spx, bq = ry * spy, c = rz * spz;
   *out = pix * sqrtf(fmaf(c, c, fmaf(bq, bq, a * a)));
+}
+
+template <typename TS, bool HW>
+__global__ void __launch_bounds__(256)
+cone_fwd_kernel(FwdProblem P, long long cell_stride) {
+  // blockIdx.z selects the batch item; items share shapes and have their own geometry
+  const int b = blockIdx.z;
+  P.cells = static_cast<const TS *>(P.cells) + (size_t)b * cell_stride * 8;
+  P.sino += (size_t)b * P.n * P.Hd * P.Wd;
+  P.inv_ar += (size_t)b * P.n * 9;
+  P.src += (size_t)b * P.n * 3;
+  P.step += b;
+  P.spacing += (size_t)b * 3;
+  fwd_block<TS, HW>(P, blockIdx.x);
+}
+
+// Up to this many problems of different shapes run in one persistent launch.
+constexpr int kMaxBatch = PYRONN_MAX_BATCHED_PROBLEMS;
+
+struct FwdBatch {
+  FwdProblem p[kMaxBatch];
+  int count;
+  long long total;
+};
+
+// Work claiming for the persistent kernels: one 64-bit counter per XCD (zeroed by
+// reset_counters_kernel before each launch).  A block on XCD x (block id mod 8; the
+// grid is a multiple of 8) claims the next item of that XCD's sequence with a global
+// atomic; the claim order decides only which workgroup computes an item, never the
+// arithmetic, so results stay bitwise identical to single launches.
+__global__ void reset_counters_kernel(unsigned long long *__restrict__ counters) {
+  if (threadIdx.x < kXcds) counters[threadIdx.x] = 0ull;
+}
+
+__device__ __forceinline__ long long claim(unsigned long long *counters, int xcd) {
+  __shared__ long long s_item;
+  if (threadIdx.x == 0) s_item = (long long)atomicAdd(counters + xcd, 1ull);
+  __syncthreads();
+  const long long k = s_item;
+  __syncthreads();
+  return k;
+}
+
+// Persistent forward projection over a work list of uneven problems.  Items keep
+// the single-launch order of each problem (problems concatenated, each a multiple of
+// 8 items); XCD x processes items x, x + 8, x + 16, ... in claim order, so the blocks
+// resident on an XCD always work on consecutive items of its detector row bands.
+template <typename TS, bool HW>
+__global__ void __launch_bounds__(256)
+cone_fwd_persistent_kernel(FwdBatch B, unsigned long long *counters) {
+  const int xcd = (int)(blockIdx.x % kXcds);
+  for (;;) {
+    const long long w = claim(counters, xcd) * kXcds + xcd;
+    if (w >= B.total) break;
+    int k = 0;
+    while (k + 1 < B.count && w >= B.p[k + 1].begin) ++k;
+    fwd_block<TS, HW>(B.p[k], w - B.p[k].begin);
+  }
 }
 
 // =====================================================================================
@@ -381,20 +444,27 @@ __device__ __forceinline__ float sample_det(const TS *__restrict__ img, int Wc, 
 // through scalar loads.  cone_backprojector_3D_CudaKernel.cu.cc:44-53 (map),
 // :84-118 (kernel); texture variant cone_backprojector_3D_CudaKernel_hardware_interp.cu.cc:55-87.
 // =====================================================================================
+struct BpProblem {
+  const void *cells;     // detector cells, [n][(H_d + 2) (W_d + 2)] x 4
+  float *vol;            // [D][H][W]
+  const float *P;        // [n][12]
+  const float *mult;     // [1]
+  const float *spacing;  // [3] (z, y, x)
+  const float *origin;   // [3] (z, y, x)
+  int W, H, D, n, Hd, Wd, tiles_x, tiles_y, zblocks;
+  long long begin;
+};
+
 template <typename TS, bool HW, int ZPT>
-__global__ void __launch_bounds__(256)
-cone_bp_kernel(const TS *__restrict__ cells, float *__restrict__ vol, const float *__restrict__ P,
-               const float *__restrict__ mult, const float *__restrict__ spacing,
-               const float *__restrict__ origin, int W, int H, int D, int n, int Hd, int Wd,
-               int zblocks) {
-  const int i = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
-  const int b = blockIdx.z / zblocks;
-  const int k0 = (blockIdx.z - b * zblocks) * ZPT;
+__device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int zb) {
+  const int i = tx * 16 + (threadIdx.x & 15);
+  const int j = ty * 16 + (threadIdx.x >> 4);
+  const int k0 = zb * ZPT;
+  const int W = Q.W, H = Q.H, D = Q.D, n = Q.n, Hd = Q.Hd, Wd = Q.Wd;
   if (i >= W || j >= H) return;
 
-  const float sx = spacing[b * 3 + 2], sy = spacing[b * 3 + 1], sz = spacing[b * 3 + 0];
-  const float ox = origin[b * 3 + 2], oy = origin[b * 3 + 1], oz = origin[b * 3 + 0];
+  const float sx = Q.spacing[2], sy = Q.spacing[1], sz = Q.spacing[0];
+  const float ox = Q.origin[2], oy = Q.origin[1], oz = Q.origin[0];
   const float x = fmaf((float)i, sx, ox), y = fmaf((float)j, sy, oy);
   float z[ZPT], acc[ZPT];
 #pragma unroll
@@ -404,12 +474,12 @@ cone_bp_kernel(const TS *__restrict__ cells, float *__restrict__ vol, const floa
   }
   const int Wc = Wd + 2;
   const size_t img_stride = (size_t)(Hd + 2) * Wc * 4;
-  const TS *img = cells + (size_t)b * n * img_stride;
-  const float *Pb = P + (size_t)b * n * 12;
+  const TS *img = static_cast<const TS *>(Q.cells);
+  const float *Pm = Q.P;
   const float fW = (float)Wd, fH = (float)Hd;
 
   for (int view = 0; view < n; ++view, img += img_stride) {
-    const float *m = Pb + view * 12;
+    const float *m = Pm + view * 12;
     const float tu = fmaf(m[1], y, m[0] * x);
     const float tv = fmaf(m[5], y, m[4] * x);
     const float tw = fmaf(m[9], y, m[8] * x);
@@ -425,10 +495,54 @@ cone_bp_kernel(const TS *__restrict__ cells, float *__restrict__ vol, const floa
       acc[q] = fmaf(f * ww, ww, acc[q]);
     }
   }
-  const float mu = mult[b];
+  const float mu = Q.mult[0];
 #pragma unroll
   for (int q = 0; q < ZPT; ++q) {
-    if (k0 + q < D) vol[(((size_t)b * D + k0 + q) * H + j) * W + i] = acc[q] * mu;
+    if (k0 + q < D) Q.vol[((size_t)(k0 + q) * H + j) * W + i] = acc[q] * mu;
+  }
+}
+
+template <typename TS, bool HW, int ZPT>
+__global__ void __launch_bounds__(256) cone_bp_kernel(BpProblem Q, size_t cells_stride) {
+  const int b = blockIdx.z / Q.zblocks;
+  const int zb = blockIdx.z - b * Q.zblocks;
+  Q.cells = static_cast<const TS *>(Q.cells) + (size_t)b * cells_stride;
+  Q.vol += (size_t)b * Q.D * Q.H * Q.W;
+  Q.P += (size_t)b * Q.n * 12;
+  Q.mult += b;
+  Q.spacing += (size_t)b * 3;
+  Q.origin += (size_t)b * 3;
+  bp_block<TS, HW, ZPT>(Q, blockIdx.x, blockIdx.y, zb);
+}
+
+struct BpBatch {
+  BpProblem p[kMaxBatch];
+  int count;
+  long long total;
+};
+
+// Persistent backprojection over uneven problems; item -> (problem, z block, tile row,
+// tile column), tile column fastest.  XCD x owns the contiguous item range
+// [x C, (x + 1) C), C = ceil(total / 8): neighbouring voxel tiles, which read
+// neighbouring detector regions of every view, share that XCD's L2.
+template <typename TS, bool HW, int ZPT>
+__global__ void __launch_bounds__(256)
+cone_bp_persistent_kernel(BpBatch B, unsigned long long *counters) {
+  const int xcd = (int)(blockIdx.x % kXcds);
+  const long long chunk = (B.total + kXcds - 1) / kXcds;
+  for (;;) {
+    const long long c = claim(counters, xcd);
+    const long long w = (long long)xcd * chunk + c;
+    if (c >= chunk || w >= B.total) break;
+    int k = 0;
+    while (k + 1 < B.count && w >= B.p[k + 1].begin) ++k;
+    const BpProblem &Q = B.p[k];
+    const long long q = w - Q.begin;
+    const long long per_z = (long long)Q.tiles_x * Q.tiles_y;
+    const int zb = (int)(q / per_z);
+    const long long r = q - (long long)zb * per_z;
+    const int ty = (int)(r / Q.tiles_x), tx = (int)(r - (long long)ty * Q.tiles_x);
+    bp_block<TS, HW, ZPT>(Q, tx, ty, zb);
   }
 }
 
@@ -462,16 +576,14 @@ template <typename TS>
 static void launch_fwd(bool hw, const void *cells, float *sino, const float *inv_ar,
                        const float *src, const float *step, const float *spacing, int B, int D,
                        int H, int W, int n, int Hd, int Wd, hipStream_t stream) {
-  const int tiles_x = (Wd + 15) / 16, tiles_y = (Hd + 15) / 16;
-  const long long blocks = fwd_blocks(n, Hd, Wd);
-  dim3 grid((unsigned)blocks, 1, B);
+  FwdProblem P{cells, sino, inv_ar, src, step, spacing, W, H, D, n, Hd, Wd,
+               (Wd + 15) / 16, (Hd + 15) / 16, 0};
+  dim3 grid((unsigned)fwd_blocks(n, Hd, Wd), 1, B);
   const long long cs = (long long)volume_cells(D, H, W);
   if (hw)
-    cone_fwd_kernel<TS, true><<<grid, dim3(256), 0, stream>>>(
-        (const TS *)cells, sino, inv_ar, src, step, spacing, W, H, D, n, Hd, Wd, tiles_x, tiles_y, cs);
+    cone_fwd_kernel<TS, true><<<grid, dim3(256), 0, stream>>>(P, cs);
   else
-    cone_fwd_kernel<TS, false><<<grid, dim3(256), 0, stream>>>(
-        (const TS *)cells, sino, inv_ar, src, step, spacing, W, H, D, n, Hd, Wd, tiles_x, tiles_y, cs);
+    cone_fwd_kernel<TS, false><<<grid, dim3(256), 0, stream>>>(P, cs);
 }
 
 // z voxels per lane: 8 for f32 detector cells, 4 for the 16-bit ones (measured
@@ -494,13 +606,14 @@ static void launch_bp(bool hw, const void *cells, float *vol, const float *P, co
                       int Hd, int Wd, hipStream_t stream) {
   constexpr int Z = bp_zpt<TS>();
   const int zblocks = (D + Z - 1) / Z;
+  BpProblem Q{cells, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd,
+              (W + 15) / 16, (H + 15) / 16, zblocks, 0};
   dim3 grid((W + 15) / 16, (H + 15) / 16, zblocks * B);
+  const size_t cs = det_cells(n, Hd, Wd) * 4;
   if (hw)
-    cone_bp_kernel<TS, true, Z><<<grid, dim3(256), 0, stream>>>(
-        (const TS *)cells, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd, zblocks);
+    cone_bp_kernel<TS, true, Z><<<grid, dim3(256), 0, stream>>>(Q, cs);
   else
-    cone_bp_kernel<TS, false, Z><<<grid, dim3(256), 0, stream>>>(
-        (const TS *)cells, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd, zblocks);
+    cone_bp_kernel<TS, false, Z><<<grid, dim3(256), 0, stream>>>(Q, cs);
 }
 
 }  // namespace pyronn
@@ -654,4 +767,197 @@ extern "C" int pyronn_cone_backprojection3d(const void *sinogram, int sinogram_d
   }
   PYRONN_CHECK_LAUNCH("cone_bp_kernel");
   return PYRONN_OK;
+}
+
+// =====================================================================================
+// Batched launches over problems of different shapes
+// =====================================================================================
+namespace pyronn {
+
+static int device_cus() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    return 256;
+  return cus;
+}
+
+static int check_problems(const char *name, const pyronn_cone_problem *pr, int count, int dtype,
+                          bool forward) {
+  PYRONN_CHECK_ARG(pr != nullptr, "%s: null problem list", name);
+  PYRONN_CHECK_ARG(count >= 1 && count <= kMaxBatch, "%s: count %d not in [1, %d]", name, count,
+                   kMaxBatch);
+  PYRONN_CHECK_ARG(dtype_ok(dtype), "%s: bad dtype %d", name, dtype);
+  for (int k = 0; k < count; ++k) {
+    const pyronn_cone_problem &p = pr[k];
+    PYRONN_CHECK_ARG(p.input && p.output && p.volume_origin && p.volume_spacing &&
+                         p.projection_matrices && (forward ? p.step_size != nullptr
+                                                           : p.projection_multiplier != nullptr),
+                     "%s: null pointer in problem %d", name, k);
+    PYRONN_CHECK_ARG(p.D > 0 && p.H > 0 && p.W > 0 && p.n_views > 0 && p.det_h > 0 && p.det_w > 0,
+                     "%s: bad shape in problem %d", name, k);
+    if (forward) {
+      if (volume_cells(p.D, p.H, p.W) >= (size_t)INT32_MAX ||
+          (long long)((p.D + 31) / 32) * p.H > 65535 ||
+          fwd_blocks(p.n_views, p.det_h, p.det_w) >= (long long)INT32_MAX) {
+        set_error("%s: problem %d too large", name, k);
+        return PYRONN_ERR_TOO_LARGE;
+      }
+    } else if ((size_t)(p.det_h + 2) * (p.det_w + 2) >= (size_t)INT32_MAX / 4) {
+      set_error("%s: problem %d detector too large", name, k);
+      return PYRONN_ERR_TOO_LARGE;
+    }
+  }
+  return PYRONN_OK;
+}
+
+static size_t fwd_problem_bytes(const pyronn_cone_problem &p, int dtype) {
+  const size_t e = dtype_bytes(dtype);
+  return fwd_geom_bytes(1, p.n_views) + align_up(volume_cells(p.D, p.H, p.W) * 8 * e, 256) +
+         align_up((size_t)p.D * p.H * p.W * e, 256);
+}
+
+constexpr size_t kCounterBytes = 256;  // per-XCD work counters at the start of a batched workspace
+
+template <typename TS>
+static int fwd_batched(const pyronn_cone_problem *pr, int count, bool hw, int bpc, char *ws,
+                       hipStream_t s) {
+  unsigned long long *counters = (unsigned long long *)ws;
+  ws += kCounterBytes;
+  FwdBatch fb{};
+  long long begin = 0;
+  for (int k = 0; k < count; ++k) {
+    const pyronn_cone_problem &p = pr[k];
+    float *inv_ar = (float *)ws;
+    float *src = inv_ar + (size_t)p.n_views * 9;
+    char *cells = ws + fwd_geom_bytes(1, p.n_views);
+    char *vt = cells + align_up(volume_cells(p.D, p.H, p.W) * 8 * sizeof(TS), 256);
+    ws += fwd_problem_bytes(p, sizeof(TS) == 4 ? PYRONN_F32 : PYRONN_BF16);
+    cone_setup_kernel<<<dim3((p.n_views + 127) / 128), dim3(128), 0, s>>>(
+        p.projection_matrices, p.volume_origin, p.volume_spacing, 1, p.n_views, inv_ar, src);
+    launch_cells<TS>(p.input, vt, cells, 1, p.D, p.H, p.W, s);
+    fb.p[k] = FwdProblem{cells, p.output, inv_ar, src, p.step_size, p.volume_spacing,
+                         p.W, p.H, p.D, p.n_views, p.det_h, p.det_w,
+                         (p.det_w + 15) / 16, (p.det_h + 15) / 16, begin};
+    begin += fwd_blocks(p.n_views, p.det_h, p.det_w);
+  }
+  fb.count = count;
+  fb.total = begin;
+  long long grid = (long long)device_cus() * (bpc > 0 ? bpc : 8);
+  grid = (grid + kXcds - 1) / kXcds * kXcds;
+  if (grid > begin) grid = begin;
+  reset_counters_kernel<<<1, 64, 0, s>>>(counters);
+  if (hw)
+    cone_fwd_persistent_kernel<TS, true><<<dim3((unsigned)grid), dim3(256), 0, s>>>(fb, counters);
+  else
+    cone_fwd_persistent_kernel<TS, false><<<dim3((unsigned)grid), dim3(256), 0, s>>>(fb, counters);
+  return PYRONN_OK;
+}
+
+template <typename TS, int Z>
+static int bp_batched(const pyronn_cone_problem *pr, int count, bool hw, int bpc, char *ws,
+                      hipStream_t s) {
+  unsigned long long *counters = (unsigned long long *)ws;
+  ws += kCounterBytes;
+  BpBatch bb{};
+  long long begin = 0;
+  for (int k = 0; k < count; ++k) {
+    const pyronn_cone_problem &p = pr[k];
+    char *cells = ws;
+    ws += align_up(det_cells(p.n_views, p.det_h, p.det_w) * 4 * sizeof(TS), 256);
+    launch_det_cells<TS>(p.input, cells, p.n_views, p.det_h, p.det_w, s);
+    const int zblocks = (p.D + Z - 1) / Z;
+    const int tx = (p.W + 15) / 16, ty = (p.H + 15) / 16;
+    bb.p[k] = BpProblem{cells, p.output, p.projection_matrices, p.projection_multiplier,
+                        p.volume_spacing, p.volume_origin, p.W, p.H, p.D, p.n_views, p.det_h,
+                        p.det_w, tx, ty, zblocks, begin};
+    begin += (long long)tx * ty * zblocks;
+  }
+  bb.count = count;
+  bb.total = begin;
+  long long grid = (long long)device_cus() * (bpc > 0 ? bpc : 8);
+  grid = (grid + kXcds - 1) / kXcds * kXcds;
+  reset_counters_kernel<<<1, 64, 0, s>>>(counters);
+  if (hw)
+    cone_bp_persistent_kernel<TS, true, Z><<<dim3((unsigned)grid), dim3(256), 0, s>>>(bb, counters);
+  else
+    cone_bp_persistent_kernel<TS, false, Z><<<dim3((unsigned)grid), dim3(256), 0, s>>>(bb, counters);
+  return PYRONN_OK;
+}
+
+}  // namespace pyronn
+
+extern "C" size_t pyronn_cone_projection3d_batched_workspace_size(const pyronn_cone_problem *problems,
+                                                                  int count, int dtype) {
+  if (!problems || count < 1 || count > kMaxBatch || !dtype_ok(dtype)) return 0;
+  size_t total = kCounterBytes;
+  for (int k = 0; k < count; ++k) total += fwd_problem_bytes(problems[k], dtype);
+  return total;
+}
+
+extern "C" int pyronn_cone_projection3d_batched(const pyronn_cone_problem *problems, int count,
+                                                int dtype, int hardware_interp, int blocks_per_cu,
+                                                void *workspace, size_t workspace_bytes,
+                                                pyronn_stream_t stream) {
+  clear_error();
+  int rc = check_problems("cone_projection3d_batched", problems, count, dtype, true);
+  if (rc != PYRONN_OK) return rc;
+  PYRONN_CHECK_ARG(workspace != nullptr, "cone_projection3d_batched: null workspace");
+  const size_t need = pyronn_cone_projection3d_batched_workspace_size(problems, count, dtype);
+  if (workspace_bytes < need) {
+    set_error("cone_projection3d_batched: workspace %zu bytes < required %zu", workspace_bytes, need);
+    return PYRONN_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const bool hw = hardware_interp != 0;
+  char *ws = (char *)workspace;
+  if (dtype == PYRONN_F32) rc = fwd_batched<float>(problems, count, hw, blocks_per_cu, ws, s);
+  else if (dtype == PYRONN_BF16) rc = fwd_batched<bf16s>(problems, count, hw, blocks_per_cu, ws, s);
+  else rc = fwd_batched<_Float16>(problems, count, hw, blocks_per_cu, ws, s);
+  PYRONN_CHECK_LAUNCH("cone_fwd_persistent_kernel");
+  return rc;
+}
+
+extern "C" size_t pyronn_cone_backprojection3d_batched_workspace_size(
+    const pyronn_cone_problem *problems, int count, int dtype) {
+  if (!problems || count < 1 || count > kMaxBatch || !dtype_ok(dtype)) return 0;
+  size_t total = kCounterBytes;
+  for (int k = 0; k < count; ++k)
+    total += align_up(det_cells(problems[k].n_views, problems[k].det_h, problems[k].det_w) * 4 *
+                          dtype_bytes(dtype),
+                      256);
+  return total;
+}
+
+extern "C" int pyronn_cone_backprojection3d_batched(const pyronn_cone_problem *problems, int count,
+                                                    int dtype, int hardware_interp, int z_per_lane,
+                                                    int blocks_per_cu, void *workspace,
+                                                    size_t workspace_bytes, pyronn_stream_t stream) {
+  clear_error();
+  int rc = check_problems("cone_backprojection3d_batched", problems, count, dtype, false);
+  if (rc != PYRONN_OK) return rc;
+  PYRONN_CHECK_ARG(workspace != nullptr, "cone_backprojection3d_batched: null workspace");
+  PYRONN_CHECK_ARG(z_per_lane <= 0 || z_per_lane == 4 || z_per_lane == 8,
+                   "cone_backprojection3d_batched: z_per_lane must be 4 or 8");
+  const size_t need = pyronn_cone_backprojection3d_batched_workspace_size(problems, count, dtype);
+  if (workspace_bytes < need) {
+    set_error("cone_backprojection3d_batched: workspace %zu bytes < required %zu", workspace_bytes,
+              need);
+    return PYRONN_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const bool hw = hardware_interp != 0;
+  const int z = z_per_lane > 0 ? z_per_lane : (dtype == PYRONN_F32 ? 8 : 4);
+  char *ws = (char *)workspace;
+  if (dtype == PYRONN_F32)
+    rc = z == 8 ? bp_batched<float, 8>(problems, count, hw, blocks_per_cu, ws, s)
+                : bp_batched<float, 4>(problems, count, hw, blocks_per_cu, ws, s);
+  else if (dtype == PYRONN_BF16)
+    rc = z == 8 ? bp_batched<bf16s, 8>(problems, count, hw, blocks_per_cu, ws, s)
+                : bp_batched<bf16s, 4>(problems, count, hw, blocks_per_cu, ws, s);
+  else
+    rc = z == 8 ? bp_batched<_Float16, 8>(problems, count, hw, blocks_per_cu, ws, s)
+                : bp_batched<_Float16, 4>(problems, count, hw, blocks_per_cu, ws, s);
+  PYRONN_CHECK_LAUNCH("cone_bp_persistent_kernel");
+  return rc;
 }

@@ -1,0 +1,94 @@
+"""Uneven batches: cone projections / backprojections of problems with different
+volume sizes, detectors and geometries in one persistent launch each
+(pyronn_cone_projection3d_batched / pyronn_cone_backprojection3d_batched).
+
+Every output equals the single-problem operator's output bit for bit; the
+batched launch only changes which workgroup computes which tile.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Sequence
+
+import numpy as np
+import torch
+
+from . import geometry as geo
+from . import native as nat
+
+_DT = {torch.float32: nat.F32, torch.bfloat16: nat.BF16, torch.float16: nat.F16}
+
+
+def _f32(a, dev):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float32).to(dev).contiguous()
+
+
+class _Geo:
+    """Device copies of one ConeGeometry's arrays (kept alive during a call)."""
+
+    def __init__(self, g: geo.ConeGeometry, dev):
+        self.g = g
+        self.P = _f32(g.projection_matrices(), dev)
+        self.vo = _f32(np.asarray(g.volume_origin), dev)
+        self.vs = _f32(np.asarray(g.volume_spacing), dev)
+        self.step = _f32([g.step_size], dev)
+        self.mult = _f32([g.projection_multiplier], dev)
+
+
+def _problems(inputs, outputs, geos: List[_Geo]):
+    arr = (nat.ConeProblem * len(inputs))()
+    for k, (x, y, gg) in enumerate(zip(inputs, outputs, geos)):
+        g = gg.g
+        D, H, W = g.volume_shape
+        Hd, Wd = g.detector_shape
+        arr[k] = nat.ConeProblem(x.data_ptr(), y.data_ptr(), gg.vo.data_ptr(), gg.vs.data_ptr(), gg.P.data_ptr(),
+                                 gg.step.data_ptr(), gg.mult.data_ptr(), D, H, W, g.n_views, Hd, Wd)
+    return arr
+
+
+def _check(n):
+    if not 1 <= n <= nat.MAX_BATCHED_PROBLEMS:
+        raise ValueError(f"1..{nat.MAX_BATCHED_PROBLEMS} problems per batched launch, got {n}")
+
+
+def cone_projection3d_batched(volumes: Sequence[torch.Tensor], geometries: Sequence[geo.ConeGeometry],
+                              hardware_interp: bool = False, blocks_per_cu: int = 0) -> List[torch.Tensor]:
+    """volumes[k] [D_k, H_k, W_k] (one storage dtype for all) -> sinograms [n_k, H_d,k, W_d,k] float32."""
+    _check(len(volumes))
+    dev = volumes[0].device
+    dt = volumes[0].dtype
+    if any(v.dtype != dt for v in volumes):
+        raise TypeError("all volumes of a batched launch share one storage dtype")
+    vols = [v.contiguous() for v in volumes]
+    geos = [_Geo(g, dev) for g in geometries]
+    outs = [torch.empty((g.n_views,) + tuple(g.detector_shape), dtype=torch.float32, device=dev) for g in geometries]
+    arr = _problems(vols, outs, geos)
+    L = nat.lib()
+    ws_bytes = L.pyronn_cone_projection3d_batched_workspace_size(arr, len(vols), _DT[dt])
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    nat.call("pyronn_cone_projection3d_batched", arr, len(vols), _DT[dt], int(bool(hardware_interp)),
+             int(blocks_per_cu), ctypes.c_void_p(ws.data_ptr()), ws_bytes,
+             ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
+    return outs
+
+
+def cone_backprojection3d_batched(sinograms: Sequence[torch.Tensor], geometries: Sequence[geo.ConeGeometry],
+                                  hardware_interp: bool = False, z_per_lane: int = 0,
+                                  blocks_per_cu: int = 0) -> List[torch.Tensor]:
+    """sinograms[k] [n_k, H_d,k, W_d,k] -> volumes [D_k, H_k, W_k] float32 (scaled by projection_multiplier)."""
+    _check(len(sinograms))
+    dev = sinograms[0].device
+    dt = sinograms[0].dtype
+    if any(s.dtype != dt for s in sinograms):
+        raise TypeError("all sinograms of a batched launch share one storage dtype")
+    sins = [s.contiguous() for s in sinograms]
+    geos = [_Geo(g, dev) for g in geometries]
+    outs = [torch.empty(tuple(g.volume_shape), dtype=torch.float32, device=dev) for g in geometries]
+    arr = _problems(sins, outs, geos)
+    L = nat.lib()
+    ws_bytes = L.pyronn_cone_backprojection3d_batched_workspace_size(arr, len(sins), _DT[dt])
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    nat.call("pyronn_cone_backprojection3d_batched", arr, len(sins), _DT[dt], int(bool(hardware_interp)),
+             int(z_per_lane), int(blocks_per_cu), ctypes.c_void_p(ws.data_ptr()), ws_bytes,
+             ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
+    return outs

@@ -14,6 +14,17 @@ HIP_LIB = os.path.join(_HERE, "lib", "libpyronn_mi.so")
 
 PYRONN_OK = 0
 F32, BF16, F16 = 0, 1, 2
+MAX_BATCHED_PROBLEMS = 16
+
+
+class ConeProblem(ctypes.Structure):
+    """struct pyronn_cone_problem (include/pyronn_mi.h)."""
+    _fields_ = [("input", ctypes.c_void_p), ("output", ctypes.c_void_p), ("volume_origin", ctypes.c_void_p),
+                ("volume_spacing", ctypes.c_void_p), ("projection_matrices", ctypes.c_void_p),
+                ("step_size", ctypes.c_void_p), ("projection_multiplier", ctypes.c_void_p),
+                ("D", ctypes.c_int), ("H", ctypes.c_int), ("W", ctypes.c_int), ("n_views", ctypes.c_int),
+                ("det_h", ctypes.c_int), ("det_w", ctypes.c_int)]
+
 
 _lib = None
 _vp = ctypes.c_void_p
@@ -39,6 +50,10 @@ _SIGS = {
                                      _vp, _vp]),
     "pyronn_fan_backprojection2d": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i,
                                          _vp, _vp]),
+    "pyronn_cone_projection3d_batched_workspace_size": (_sz, [_vp, _i, _i]),
+    "pyronn_cone_projection3d_batched": (_i, [_vp, _i, _i, _i, _i, _vp, _sz, _vp]),
+    "pyronn_cone_backprojection3d_batched_workspace_size": (_sz, [_vp, _i, _i]),
+    "pyronn_cone_backprojection3d_batched": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _sz, _vp]),
     "pyronn_fdk_ramp_matrix": (_i, [_i, ctypes.c_double, _vp, _vp, _vp]),
     "pyronn_fdk_cosine_weights": (_i, [_i, _i] + [ctypes.c_double] * 5 + [_vp, _vp]),
     "pyronn_fdk_filter_workspace_size": (_sz, [_i, _i, _i]),

@@ -99,3 +99,21 @@ def test_torch_ops_registered_and_cpu_tensors_rejected():
     with pytest.raises(Exception):
         pl.cone_projection3d(vol, [2, 4, 4], torch.zeros(1, 3), torch.ones(1, 3), torch.zeros(1, 2, 3, 4),
                              torch.ones(1, 1), torch.ones(1, 1))
+
+
+def test_batched_argument_validation():
+    L = nat.lib()
+    P = nat.ConeProblem
+    good = P(0x1000, 0x2000, 0x3000, 0x4000, 0x5000, 0x6000, 0x7000, 4, 4, 4, 2, 4, 4)
+    arr = (P * 17)(*([good] * 17))
+    assert L.pyronn_cone_projection3d_batched(arr, 0, 0, 0, 0, _fake(), 1 << 30, None) == 1
+    assert "count" in nat.last_error()
+    assert L.pyronn_cone_projection3d_batched(arr, 17, 0, 0, 0, _fake(), 1 << 30, None) == 1
+    bad = (P * 2)(good, P(0, 0x2000, 0x3000, 0x4000, 0x5000, 0x6000, 0x7000, 4, 4, 4, 2, 4, 4))
+    assert L.pyronn_cone_projection3d_batched(bad, 2, 0, 0, 0, _fake(), 1 << 30, None) == 1
+    assert "problem 1" in nat.last_error()
+    assert L.pyronn_cone_backprojection3d_batched(arr, 2, 0, 0, 5, 0, _fake(), 1 << 30, None) == 1
+    assert "z_per_lane" in nat.last_error()
+    need = L.pyronn_cone_projection3d_batched_workspace_size(arr, 2, 0)
+    assert need == 256 + 2 * L.pyronn_cone_projection3d_workspace_size(1, 4, 4, 4, 2, 0)
+    assert L.pyronn_cone_projection3d_batched(arr, 2, 0, 0, 0, _fake(), need - 1, None) == 2
