@@ -2,8 +2,8 @@
 //
 // Kernels
 //   cone_setup_kernel   per-view FP64 geometry (M^-1, camera centre) on the device
-//   build_cells_kernel  copy of the volume as 2x2x2 neighbourhood cells with a zero
-//                       border (what the forward sampler reads)
+//   build_cells_kernel  copy of the volume as neighbourhood cells with a zero border
+//                       (what the forward sampler reads)
 //   build_det_cells_kernel  copy of the sinogram as 2x2 neighbourhood cells with a
 //                       zero border (what the backprojection sampler reads)
 //   cone_fwd_kernel     ray-driven forward projection, one lane per detector pixel
@@ -61,20 +61,33 @@ __global__ void cone_setup_kernel(const float *__restrict__ P, const float *__re
 }
 
 // =====================================================================================
-// Neighbourhood cells.  The forward projector reads the volume through a copy in
-// which cell (x, y, z), x in [-1, W], y in [-1, H], z in [-1, D], holds the eight
-// voxels of its trilinear neighbourhood
-//   [v(x,y,z), v(x+1,y,z), v(x,y+1,z), v(x+1,y+1,z), (same at z+1)]
-// with zeros outside the volume.  One sample then costs one 16-byte load (bf16 /
-// f16 storage) or two (f32) instead of eight 4-byte gathers, and the zero border
-// makes the reference's per-neighbour bounds tests unnecessary.
-// Cells are stored z-fastest: index ((x+1) (H+2) + (y+1)) (D+2) + (z+1).  For an
-// orbit about the z axis, z is transverse to every ray, so the 64 rays of a wave
-// (8 detector rows ~ 8 z positions) fall into few cache lines for every view
-// direction.  Other trajectories are handled identically, only less cache-friendly.
+// Neighbourhood cells.  The forward projector reads the volume through a copy made
+// of cells that hold a voxel neighbourhood, with zeros outside the volume, stored
+// z-fastest so that the 64 rays of a wave (8 detector rows ~ 8 z positions, and z
+// is transverse to every ray of an orbit about the z axis) fall into few cache lines
+// for every view direction.  The zero border makes the reference's per-neighbour
+// bounds tests unnecessary.  Two layouts, chosen by storage type (measured on the
+// 512^3 / 720-view workload):
+//  * f32 - xy quads: cell (x, y, z), x in [-1, W], y in [-1, H], z in [-1, D + 1],
+//    holds [v(x,y,z), v(x+1,y,z), v(x,y+1,z), v(x+1,y+1,z)] (16 bytes) at index
+//    ((x+1) (H+2) + (y+1)) (D+3) + (z+1).  A sample reads cells c and c + 1 (the z
+//    and z+1 quads), two adjacent 16-byte loads sharing cache lines.
+//  * bf16 / f16 - 2x2x2 cells: cell (x, y, z), x, y, z in [-1, N], holds the quad at
+//    z followed by the quad at z + 1 (16 bytes) at ((x+1) (H+2) + (y+1)) (D+2) + (z+1).
+//    A sample is one 16-byte load.
 // The build goes through a z-fastest transpose of the volume so that both of its
 // passes read and write contiguously.
 // =====================================================================================
+template <typename TS> struct CellLayout {
+  static constexpr bool kQuad = sizeof(TS) == 4;  // f32: xy quads; 16-bit: 2x2x2 cells
+  static constexpr int kPerCell = kQuad ? 4 : 8;  // elements per cell
+  static constexpr int kZExtra = kQuad ? 3 : 2;   // cells along z beyond D
+};
+
+template <typename TS> static size_t cell_elems(int D, int H, int W) {
+  return (size_t)(W + 2) * (H + 2) * (D + CellLayout<TS>::kZExtra) * CellLayout<TS>::kPerCell;
+}
+
 template <typename TS>
 __global__ void __launch_bounds__(256)
 transpose_zx_kernel(const TS *__restrict__ vol, TS *__restrict__ vt, int W, int H, int D) {
@@ -110,7 +123,8 @@ __device__ __forceinline__ TS voxt(const TS *__restrict__ vt, int x, int y, int 
 template <typename TS>
 __global__ void build_cells_kernel(const TS *__restrict__ vt, TS *__restrict__ cells, int W,
                                    int H, int D, long long total) {
-  const int Wc = W + 2, Hc = H + 2, Dc = D + 2;
+  using L = CellLayout<TS>;
+  const int Wc = W + 2, Hc = H + 2, Dc = D + L::kZExtra;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
     long long r = t;
@@ -125,13 +139,15 @@ __global__ void build_cells_kernel(const TS *__restrict__ vt, TS *__restrict__ c
     c[1] = voxt(v, x + 1, y, z, W, H, D);
     c[2] = voxt(v, x, y + 1, z, W, H, D);
     c[3] = voxt(v, x + 1, y + 1, z, W, H, D);
-    c[4] = voxt(v, x, y, z + 1, W, H, D);
-    c[5] = voxt(v, x + 1, y, z + 1, W, H, D);
-    c[6] = voxt(v, x, y + 1, z + 1, W, H, D);
-    c[7] = voxt(v, x + 1, y + 1, z + 1, W, H, D);
-    TS *o = cells + (size_t)t * 8;
+    if (!L::kQuad) {
+      c[4] = voxt(v, x, y, z + 1, W, H, D);
+      c[5] = voxt(v, x + 1, y, z + 1, W, H, D);
+      c[6] = voxt(v, x, y + 1, z + 1, W, H, D);
+      c[7] = voxt(v, x + 1, y + 1, z + 1, W, H, D);
+    }
+    TS *o = cells + (size_t)t * L::kPerCell;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) o[q] = c[q];
+    for (int q = 0; q < L::kPerCell; ++q) o[q] = c[q];
   }
 }
 
@@ -149,7 +165,7 @@ __device__ __forceinline__ void unpack2(uint32_t w, _Float16, float &a, float &b
 // Trilinear sample at (px, py, pz), lerp z -> y -> x exactly as
 // cone_projector_3D_CudaKernel.cu.cc:74-82.  The coordinate is clamped into the
 // zero border first (identical results for every finite coordinate).
-// Dc = D + 2, HDc = (H + 2) * (D + 2): strides of the z-fastest cell array.
+// Dc = cells along z, HDc = (H + 2) * Dc: strides of the z-fastest cell array.
 template <typename TS>
 __device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int Dc, int HDc,
                                               float px, float py, float pz, float fW, float fH,
@@ -161,9 +177,9 @@ __device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int 
   const float xd = px - fx, yd = py - fy, zd = pz - fz;
   const unsigned cell = (unsigned)(((int)fx + 1) * HDc + ((int)fy + 1) * Dc + ((int)fz + 1));
   float p000, p001, p010, p011, p100, p101, p110, p111;
-  if constexpr (sizeof(TS) == 4) {
-    const float4 *c4 = reinterpret_cast<const float4 *>(cells) + (size_t)cell * 2;
-    const float4 a = c4[0], b = c4[1];
+  if constexpr (CellLayout<TS>::kQuad) {
+    const float4 *c4 = reinterpret_cast<const float4 *>(cells) + cell;
+    const float4 a = c4[0], b = c4[1];  // quads at z and z + 1
     p000 = a.x; p001 = a.y; p010 = a.z; p011 = a.w;
     p100 = b.x; p101 = b.y; p110 = b.z; p111 = b.w;
   } else {
@@ -241,7 +257,7 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   const float step = P.step[0];
   const float spx = P.spacing[2], spy = P.spacing[1], spz = P.spacing[0];
   const TS *v = static_cast<const TS *>(P.cells);
-  const int Wp = D + 2, HWp = (H + 2) * Wp;  // z-fastest cell strides
+  const int Wp = D + CellLayout<TS>::kZExtra, HWp = (H + 2) * Wp;  // z-fastest cell strides
   const float fW = (float)W, fH = (float)H, fD = (float)D;
   float *out = P.sino + ((size_t)view * Hd + row) * (size_t)Wd + col;
   const float u = (float)col, w = (float)row;
@@ -319,7 +335,7 @@ __global__ void __launch_bounds__(256)
 cone_fwd_kernel(FwdProblem P, long long cell_stride) {
   // blockIdx.z selects the batch item; items share shapes and have their own geometry
   const int b = blockIdx.z;
-  P.cells = static_cast<const TS *>(P.cells) + (size_t)b * cell_stride * 8;
+  P.cells = static_cast<const TS *>(P.cells) + (size_t)b * cell_stride;
   P.sino += (size_t)b * P.n * P.Hd * P.Wd;
   P.inv_ar += (size_t)b * P.n * 9;
   P.src += (size_t)b * P.n * 3;
@@ -522,18 +538,17 @@ struct BpBatch {
 };
 
 // Persistent backprojection over uneven problems; item -> (problem, z block, tile row,
-// tile column), tile column fastest.  XCD x owns the contiguous item range
-// [x C, (x + 1) C), C = ceil(total / 8): neighbouring voxel tiles, which read
-// neighbouring detector regions of every view, share that XCD's L2.
+// tile column), tile column fastest.  XCD x takes items x, x + 8, ... in claim order,
+// so at any time all XCDs work on neighbouring tiles of the same z block and share
+// that block's band of detector rows (of every view) in the Infinity Cache - the
+// order of the single-problem grid.
 template <typename TS, bool HW, int ZPT>
 __global__ void __launch_bounds__(256)
 cone_bp_persistent_kernel(BpBatch B, unsigned long long *counters) {
   const int xcd = (int)(blockIdx.x % kXcds);
-  const long long chunk = (B.total + kXcds - 1) / kXcds;
   for (;;) {
-    const long long c = claim(counters, xcd);
-    const long long w = (long long)xcd * chunk + c;
-    if (c >= chunk || w >= B.total) break;
+    const long long w = claim(counters, xcd) * kXcds + xcd;
+    if (w >= B.total) break;
     int k = 0;
     while (k + 1 < B.count && w >= B.p[k + 1].begin) ++k;
     const BpProblem &Q = B.p[k];
@@ -549,8 +564,8 @@ cone_bp_persistent_kernel(BpBatch B, unsigned long long *counters) {
 // =====================================================================================
 // Host side
 // =====================================================================================
-static size_t volume_cells(int D, int H, int W) {
-  return (size_t)(D + 2) * (H + 2) * (W + 2);
+static size_t cell_elems_dt(int dtype, int D, int H, int W) {
+  return dtype == PYRONN_F32 ? cell_elems<float>(D, H, W) : cell_elems<bf16s>(D, H, W);
 }
 static size_t det_cells(int n, int Hd, int Wd) { return (size_t)n * (Hd + 2) * (Wd + 2); }
 
@@ -565,7 +580,8 @@ static void launch_cells(const void *vol, void *vt, void *cells, int B, int D, i
                          hipStream_t stream) {
   dim3 tgrid((W + 31) / 32, ((D + 31) / 32) * H, B);
   transpose_zx_kernel<TS><<<tgrid, dim3(256), 0, stream>>>((const TS *)vol, (TS *)vt, W, H, D);
-  const long long total = (long long)B * volume_cells(D, H, W);
+  const long long total =
+      (long long)B * (long long)(cell_elems<TS>(D, H, W) / CellLayout<TS>::kPerCell);
   long long blocks = (total + 255) / 256;
   if (blocks > 65536 * 4) blocks = 65536 * 4;
   build_cells_kernel<TS><<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(
@@ -579,7 +595,7 @@ static void launch_fwd(bool hw, const void *cells, float *sino, const float *inv
   FwdProblem P{cells, sino, inv_ar, src, step, spacing, W, H, D, n, Hd, Wd,
                (Wd + 15) / 16, (Hd + 15) / 16, 0};
   dim3 grid((unsigned)fwd_blocks(n, Hd, Wd), 1, B);
-  const long long cs = (long long)volume_cells(D, H, W);
+  const long long cs = (long long)cell_elems<TS>(D, H, W);
   if (hw)
     cone_fwd_kernel<TS, true><<<grid, dim3(256), 0, stream>>>(P, cs);
   else
@@ -642,7 +658,7 @@ extern "C" size_t pyronn_cone_projection3d_workspace_size(int batch, int D, int 
                                                           int n_views, int volume_dtype) {
   if (batch <= 0 || D <= 0 || H <= 0 || W <= 0 || n_views <= 0 || !dtype_ok(volume_dtype)) return 0;
   return fwd_geom_bytes(batch, n_views) +
-         align_up((size_t)batch * volume_cells(D, H, W) * 8 * dtype_bytes(volume_dtype), 256) +
+         align_up((size_t)batch * cell_elems_dt(volume_dtype, D, H, W) * dtype_bytes(volume_dtype), 256) +
          align_up((size_t)batch * D * H * W * dtype_bytes(volume_dtype), 256);
 }
 
@@ -664,7 +680,7 @@ extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, in
                        n_views <= 65535 && det_h > 0 && det_w > 0,
                    "cone_projection3d: bad shape B=%d D=%d H=%d W=%d n=%d det=%dx%d", batch, D, H, W,
                    n_views, det_h, det_w);
-  if (volume_cells(D, H, W) >= (size_t)INT32_MAX) {
+  if (cell_elems_dt(volume_dtype, D, H, W) / 4 >= (size_t)INT32_MAX) {
     set_error("cone_projection3d: volume %dx%dx%d exceeds the 2^31-cell limit", D, H, W);
     return PYRONN_ERR_TOO_LARGE;
   }
@@ -687,7 +703,7 @@ extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, in
   float *srcp = inv_ar + (size_t)batch * n_views * 9;
   void *cells = ws + fwd_geom_bytes(batch, n_views);
   void *vt = (char *)cells +
-             align_up((size_t)batch * volume_cells(D, H, W) * 8 * dtype_bytes(volume_dtype), 256);
+             align_up((size_t)batch * cell_elems_dt(volume_dtype, D, H, W) * dtype_bytes(volume_dtype), 256);
 
   int rc = pyronn_cone_geometry_setup(projection_matrices, volume_origin, volume_spacing, batch,
                                       n_views, inv_ar, srcp, stream);
@@ -797,7 +813,7 @@ static int check_problems(const char *name, const pyronn_cone_problem *pr, int c
     PYRONN_CHECK_ARG(p.D > 0 && p.H > 0 && p.W > 0 && p.n_views > 0 && p.det_h > 0 && p.det_w > 0,
                      "%s: bad shape in problem %d", name, k);
     if (forward) {
-      if (volume_cells(p.D, p.H, p.W) >= (size_t)INT32_MAX ||
+      if (cell_elems_dt(dtype, p.D, p.H, p.W) / 4 >= (size_t)INT32_MAX ||
           (long long)((p.D + 31) / 32) * p.H > 65535 ||
           fwd_blocks(p.n_views, p.det_h, p.det_w) >= (long long)INT32_MAX) {
         set_error("%s: problem %d too large", name, k);
@@ -813,7 +829,7 @@ static int check_problems(const char *name, const pyronn_cone_problem *pr, int c
 
 static size_t fwd_problem_bytes(const pyronn_cone_problem &p, int dtype) {
   const size_t e = dtype_bytes(dtype);
-  return fwd_geom_bytes(1, p.n_views) + align_up(volume_cells(p.D, p.H, p.W) * 8 * e, 256) +
+  return fwd_geom_bytes(1, p.n_views) + align_up(cell_elems_dt(dtype, p.D, p.H, p.W) * e, 256) +
          align_up((size_t)p.D * p.H * p.W * e, 256);
 }
 
@@ -831,7 +847,7 @@ static int fwd_batched(const pyronn_cone_problem *pr, int count, bool hw, int bp
     float *inv_ar = (float *)ws;
     float *src = inv_ar + (size_t)p.n_views * 9;
     char *cells = ws + fwd_geom_bytes(1, p.n_views);
-    char *vt = cells + align_up(volume_cells(p.D, p.H, p.W) * 8 * sizeof(TS), 256);
+    char *vt = cells + align_up(cell_elems<TS>(p.D, p.H, p.W) * sizeof(TS), 256);
     ws += fwd_problem_bytes(p, sizeof(TS) == 4 ? PYRONN_F32 : PYRONN_BF16);
     cone_setup_kernel<<<dim3((p.n_views + 127) / 128), dim3(128), 0, s>>>(
         p.projection_matrices, p.volume_origin, p.volume_spacing, 1, p.n_views, inv_ar, src);
