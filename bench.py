@@ -188,6 +188,11 @@ def main(argv=None):
             "achieved": round(B_dom / t_dom / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": round(B_dom / t_dom / HBM_PEAK, 4), "traffic": traffic,
             "traffic_source": "profiles/cone_m1_f32_traffic.json (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
+            # measured HBM bytes over the live duration: the share of HBM bandwidth actually used
+            "traffic_frac": round(traffic / t_dom / HBM_PEAK, 4) if traffic else None,
+            "limiter": ("vector-memory address path (TA 98% busy, VALU 73%; profiles/cone_m1_f32_counters.md): "
+                        "algorithmic bytes exceed HBM peak because the L1/L2 serve ~90% of corner reads")
+            if dominant == "cone_fwd_kernel" else None,
         },
         "fwd_back": {
             "t_fwd_ms": round(tf * 1e3, 2), "t_bp_ms": round(tb * 1e3, 2),

@@ -22,6 +22,8 @@ INCLUDE = os.path.join(REPO, "include")
 OUT_DIR = os.path.join(PKG_ROOT, "pyronn_layers", "lib")
 HIP_LIB = os.path.join(OUT_DIR, "libpyronn_mi.so")
 OPS_LIB = os.path.join(OUT_DIR, "libpyronn_torch_ops.so")
+TESTKIT_SRC = os.path.join(REPO, "tests", "native", "rcp_check.hip")
+TESTKIT_LIB = os.path.join(REPO, "tests", "native", "libpyronn_testkit.so")
 
 HIP_SOURCES = ["abi_common.cpp", "cone.hip", "planar.hip", "fdk.hip"]
 HIP_FLAGS = [
@@ -66,6 +68,17 @@ def build_hip(verbose=False) -> str:
     return HIP_LIB
 
 
+def build_testkit(verbose=False) -> str:
+    """Device-side checks used only by the test suite (tests/native), built with the
+    library's flags so that they exercise the same code generation."""
+    tmp = TESTKIT_LIB + ".tmp"
+    dt, _ = _run([hipcc()] + HIP_FLAGS + ["-shared", "-I", INCLUDE, "-I", CSRC, TESTKIT_SRC, "-o", tmp])
+    if verbose:
+        print(f"  rcp_check.hip: {dt:.1f}s")
+    os.replace(tmp, TESTKIT_LIB)
+    return TESTKIT_LIB
+
+
 def build_ops(verbose=False) -> str:
     """g++ build of the TORCH_LIBRARY op library against libtorch and libpyronn_mi."""
     import torch
@@ -95,6 +108,8 @@ def main(argv=None):
     ap.add_argument("-v", "--verbose", action="store_true")
     a = ap.parse_args(argv)
     print("building", build_hip(a.verbose))
+    if os.path.exists(TESTKIT_SRC):
+        print("building", build_testkit(a.verbose))
     if not a.hip_only and os.path.exists(os.path.join(CSRC, "torch_ops.cpp")):
         print("building", build_ops(a.verbose))
 

@@ -162,37 +162,61 @@ __device__ __forceinline__ void unpack2(uint32_t w, _Float16, float &a, float &b
   b = (float)h[1];
 }
 
+// The eight corners of the cell a lane read last.  Consecutive samples of a ray are
+// half a voxel apart, so about half of them fall in the cell of the sample before;
+// those reuse the registers instead of loading again.  The blend is the same
+// arithmetic on the same values either way.  2^32 - 1 is never a cell index (the
+// host keeps the cell count below 2^31), so the first sample of a ray always loads.
+struct CellRegs {
+  unsigned cell = 0xffffffffu;
+  float c[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+};
+
+// Clamp of a finite coordinate into the zero border, one v_med3 instead of a max and a
+// min.  Forward sample coordinates are finite: the ray is clipped against the volume
+// box, and at least one direction component has magnitude >= 1/sqrt(3), so the march
+// parameter stays finite and the loop never runs on a NaN parameter.
+__device__ __forceinline__ float clamp_apron_finite(float c, float n) {
+  return __builtin_amdgcn_fmed3f(c, -1.0f, n);
+}
+
 // Trilinear sample at (px, py, pz), lerp z -> y -> x exactly as
 // cone_projector_3D_CudaKernel.cu.cc:74-82.  The coordinate is clamped into the
 // zero border first (identical results for every finite coordinate).
-// Dc = cells along z, HDc = (H + 2) * Dc: strides of the z-fastest cell array.
+// Dc = cells along z, HDc = (H + 2) * Dc: strides of the z-fastest cell array.  The
+// index uses signed 24-bit multiplies (full rate); the host keeps W + 2 and HDc below
+// 2^23 and the cell count below 2^31.
 template <typename TS>
 __device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int Dc, int HDc,
                                               float px, float py, float pz, float fW, float fH,
-                                              float fD) {
-  px = clamp_apron(px, fW);
-  py = clamp_apron(py, fH);
-  pz = clamp_apron(pz, fD);
+                                              float fD, CellRegs &r) {
+  px = clamp_apron_finite(px, fW);
+  py = clamp_apron_finite(py, fH);
+  pz = clamp_apron_finite(pz, fD);
   const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
   const float xd = px - fx, yd = py - fy, zd = pz - fz;
-  const unsigned cell = (unsigned)(((int)fx + 1) * HDc + ((int)fy + 1) * Dc + ((int)fz + 1));
-  float p000, p001, p010, p011, p100, p101, p110, p111;
-  if constexpr (CellLayout<TS>::kQuad) {
-    const float4 *c4 = reinterpret_cast<const float4 *>(cells) + cell;
-    const float4 a = c4[0], b = c4[1];  // quads at z and z + 1
-    p000 = a.x; p001 = a.y; p010 = a.z; p011 = a.w;
-    p100 = b.x; p101 = b.y; p110 = b.z; p111 = b.w;
-  } else {
-    const uint4 q = reinterpret_cast<const uint4 *>(cells)[cell];
-    unpack2(q.x, TS{}, p000, p001);
-    unpack2(q.y, TS{}, p010, p011);
-    unpack2(q.z, TS{}, p100, p101);
-    unpack2(q.w, TS{}, p110, p111);
+  const unsigned cell =
+      (unsigned)(__mul24((int)fx, HDc) + __mul24((int)fy, Dc) + (int)fz + (HDc + Dc + 1));
+  if (cell != r.cell) {
+    r.cell = cell;
+    if constexpr (CellLayout<TS>::kQuad) {
+      const float4 *c4 = reinterpret_cast<const float4 *>(cells) + cell;
+      const float4 a = c4[0], b = c4[1];  // quads at z and z + 1
+      r.c[0] = a.x; r.c[1] = a.y; r.c[2] = a.z; r.c[3] = a.w;
+      r.c[4] = b.x; r.c[5] = b.y; r.c[6] = b.z; r.c[7] = b.w;
+    } else {
+      const uint4 q = reinterpret_cast<const uint4 *>(cells)[cell];
+      unpack2(q.x, TS{}, r.c[0], r.c[1]);
+      unpack2(q.y, TS{}, r.c[2], r.c[3]);
+      unpack2(q.z, TS{}, r.c[4], r.c[5]);
+      unpack2(q.w, TS{}, r.c[6], r.c[7]);
+    }
   }
-  const float p00 = lerp_ref(p000, p100, zd);
-  const float p01 = lerp_ref(p001, p101, zd);
-  const float p10 = lerp_ref(p010, p110, zd);
-  const float p11 = lerp_ref(p011, p111, zd);
+  // r.c = p000 p001 p010 p011 (z) then the same four at z + 1; digits are z, y, x.
+  const float p00 = lerp_ref(r.c[0], r.c[4], zd);
+  const float p01 = lerp_ref(r.c[1], r.c[5], zd);
+  const float p10 = lerp_ref(r.c[2], r.c[6], zd);
+  const float p11 = lerp_ref(r.c[3], r.c[7], zd);
   const float p0 = lerp_ref(p00, p10, yd);
   const float p1 = lerp_ref(p01, p11, yd);
   return lerp_ref(p0, p1, xd);
@@ -200,13 +224,14 @@ __device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int 
 
 template <typename TS, bool HW>
 __device__ __forceinline__ float cone_sample(const TS *__restrict__ c, int Dc, int HDc, float px,
-                                             float py, float pz, float fW, float fH, float fD) {
+                                             float py, float pz, float fW, float fH, float fD,
+                                             CellRegs &r) {
   if (HW) {  // tex3D(p + 0.5): the sampler subtracts the half texel again
     px = (px + 0.5f) - 0.5f;
     py = (py + 0.5f) - 0.5f;
     pz = (pz + 0.5f) - 0.5f;
   }
-  return sample_cells(c, Dc, HDc, px, py, pz, fW, fH, fD);
+  return sample_cells(c, Dc, HDc, px, py, pz, fW, fH, fD, r);
 }
 
 // =====================================================================================
@@ -299,24 +324,25 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   amin = fmaf(step, 0.5f, amin);
 
   float pix = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
+  CellRegs regs;
   const int cap = march_cap(amin, amax, step);
   if (amin < amax) {
     px = fmaf(amin, rx, s0);
     py = fmaf(amin, ry, s1);
     pz = fmaf(amin, rz, s2);
-    pix = fmaf(0.5f, cone_sample<TS, HW>(v, Wp, HWp, px, py, pz, fW, fH, fD), pix);
+    pix = fmaf(0.5f, cone_sample<TS, HW>(v, Wp, HWp, px, py, pz, fW, fH, fD, regs), pix);
     amin += step;
   }
   for (int it = 0; amin < amax && it < cap; ++it) {
     px = fmaf(amin, rx, s0);
     py = fmaf(amin, ry, s1);
     pz = fmaf(amin, rz, s2);
-    pix += cone_sample<TS, HW>(v, Wp, HWp, px, py, pz, fW, fH, fD);
+    pix += cone_sample<TS, HW>(v, Wp, HWp, px, py, pz, fW, fH, fD, regs);
     amin += step;
   }
   pix *= step;
   if (pix > 0.0f) {  // tail only for a positive sum (cone_projector_3D_CudaKernel.cu.cc:164-179)
-    const float flast = cone_sample<TS, HW>(v, Wp, HWp, px, py, pz, fW, fH, fD);
+    const float flast = cone_sample<TS, HW>(v, Wp, HWp, px, py, pz, fW, fH, fD, regs);
     pix = fmaf(-(0.5f * step), flast, pix);
     amin -= step;
     const float last = amax - amin;
@@ -324,7 +350,7 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
     px = fmaf(amax, rx, s0);
     py = fmaf(amax, ry, s1);
     pz = fmaf(amax, rz, s2);
-    pix = fmaf(0.5f * last, cone_sample<TS, HW>(v, Wp, HWp, px, py, pz, fW, fH, fD), pix);
+    pix = fmaf(0.5f * last, cone_sample<TS, HW>(v, Wp, HWp, px, py, pz, fW, fH, fD, regs), pix);
   }
   const float a = rx * spx, bq = ry * spy, c = rz * spz;
   *out = pix * sqrtf(fmaf(c, c, fmaf(bq, bq, a * a)));
@@ -437,7 +463,8 @@ __device__ __forceinline__ float sample_det(const TS *__restrict__ img, int Wc, 
   v = clamp_apron(v, fH);
   const float fx = floorf(u), fy = floorf(v);
   const float xd = u - fx, yd = v - fy;
-  const unsigned c = (unsigned)(((int)fy + 1) * Wc + ((int)fx + 1));
+  // signed 24-bit multiply (full rate); the host keeps W_d + 2 and H_d + 2 below 2^23
+  const unsigned c = (unsigned)(__mul24((int)fy, Wc) + (int)fx + (Wc + 1));
   float p00, p01, p10, p11;
   if constexpr (sizeof(TS) == 4) {
     const float4 a = reinterpret_cast<const float4 *>(img)[c];
@@ -504,7 +531,7 @@ __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int
       float uu = fmaf(m[2], z[q], tu) + m[3];
       float vv = fmaf(m[6], z[q], tv) + m[7];
       float ww = fmaf(m[10], z[q], tw) + m[11];
-      ww = 1.0f / ww;
+      ww = rcp_rn(ww);  // correctly rounded, as the reference's division
       uu *= ww;
       vv *= ww;
       const float f = sample_det<TS, HW>(img, Wc, uu, vv, fW, fH);
@@ -566,6 +593,14 @@ cone_bp_persistent_kernel(BpBatch B, unsigned long long *counters) {
 // =====================================================================================
 static size_t cell_elems_dt(int dtype, int D, int H, int W) {
   return dtype == PYRONN_F32 ? cell_elems<float>(D, H, W) : cell_elems<bf16s>(D, H, W);
+}
+
+// Limits of the forward's index arithmetic: fewer than 2^31 cells, and W + 2 and the
+// (y, z) plane of cells below 2^23 for the 24-bit multiplies of the sampler.
+static bool fwd_volume_fits(int dtype, int D, int H, int W) {
+  const int zx = dtype == PYRONN_F32 ? CellLayout<float>::kZExtra : CellLayout<bf16s>::kZExtra;
+  return cell_elems_dt(dtype, D, H, W) / 4 < (size_t)INT32_MAX && W + 2 < (1 << 23) &&
+         (long long)(H + 2) * (D + zx) < (1LL << 23);
 }
 static size_t det_cells(int n, int Hd, int Wd) { return (size_t)n * (Hd + 2) * (Wd + 2); }
 
@@ -680,8 +715,8 @@ extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, in
                        n_views <= 65535 && det_h > 0 && det_w > 0,
                    "cone_projection3d: bad shape B=%d D=%d H=%d W=%d n=%d det=%dx%d", batch, D, H, W,
                    n_views, det_h, det_w);
-  if (cell_elems_dt(volume_dtype, D, H, W) / 4 >= (size_t)INT32_MAX) {
-    set_error("cone_projection3d: volume %dx%dx%d exceeds the 2^31-cell limit", D, H, W);
+  if (!fwd_volume_fits(volume_dtype, D, H, W)) {
+    set_error("cone_projection3d: volume %dx%dx%d exceeds the cell index limits", D, H, W);
     return PYRONN_ERR_TOO_LARGE;
   }
   if ((long long)((D + 31) / 32) * H > 65535) {
@@ -755,7 +790,8 @@ extern "C" int pyronn_cone_backprojection3d(const void *sinogram, int sinogram_d
                    H, W, n_views, det_h, det_w);
   PYRONN_CHECK_ARG((long long)((D + 3) / 4) * batch <= 65535,
                    "cone_backprojection3d: batch x depth too large for one launch");
-  if ((size_t)(det_h + 2) * (det_w + 2) >= (size_t)INT32_MAX / 4) {
+  if ((size_t)(det_h + 2) * (det_w + 2) >= (size_t)INT32_MAX / 4 || det_w + 2 >= (1 << 23) ||
+      det_h + 2 >= (1 << 23)) {
     set_error("cone_backprojection3d: detector too large");
     return PYRONN_ERR_TOO_LARGE;
   }
@@ -813,13 +849,14 @@ static int check_problems(const char *name, const pyronn_cone_problem *pr, int c
     PYRONN_CHECK_ARG(p.D > 0 && p.H > 0 && p.W > 0 && p.n_views > 0 && p.det_h > 0 && p.det_w > 0,
                      "%s: bad shape in problem %d", name, k);
     if (forward) {
-      if (cell_elems_dt(dtype, p.D, p.H, p.W) / 4 >= (size_t)INT32_MAX ||
+      if (!fwd_volume_fits(dtype, p.D, p.H, p.W) ||
           (long long)((p.D + 31) / 32) * p.H > 65535 ||
           fwd_blocks(p.n_views, p.det_h, p.det_w) >= (long long)INT32_MAX) {
         set_error("%s: problem %d too large", name, k);
         return PYRONN_ERR_TOO_LARGE;
       }
-    } else if ((size_t)(p.det_h + 2) * (p.det_w + 2) >= (size_t)INT32_MAX / 4) {
+    } else if ((size_t)(p.det_h + 2) * (p.det_w + 2) >= (size_t)INT32_MAX / 4 ||
+               p.det_w + 2 >= (1 << 23) || p.det_h + 2 >= (1 << 23)) {
       set_error("%s: problem %d detector too large", name, k);
       return PYRONN_ERR_TOO_LARGE;
     }

@@ -74,6 +74,14 @@ def test_too_large_volume_rejected():
     rc = L.pyronn_cone_projection3d(_fake(), 0, 1, 2048, 2048, 2048, _fake(), _fake(), _fake(), _fake(), _fake(), 2,
                                     4, 4, 0, _fake(), _fake(), 1 << 62, None)
     assert rc == 3
+    # few cells, but a row longer than the sampler's 24-bit index arithmetic allows
+    rc = L.pyronn_cone_projection3d(_fake(), 0, 1, 1, 1, (1 << 23) - 1, _fake(), _fake(), _fake(), _fake(), _fake(),
+                                    2, 4, 4, 0, _fake(), _fake(), 1 << 62, None)
+    assert rc == 3 and "index limits" in nat.last_error()
+    # the largest row that fits is accepted up to the workspace check
+    rc = L.pyronn_cone_projection3d(_fake(), 0, 1, 1, 1, (1 << 23) - 3, _fake(), _fake(), _fake(), _fake(), _fake(),
+                                    2, 4, 4, 0, _fake(), _fake(), 16, None)
+    assert rc == 2
 
 
 def test_workspace_sizes_scale_with_storage():
