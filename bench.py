@@ -52,6 +52,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-storage run")
+    ap.add_argument("--no-hw", action="store_true", help="skip the hardware_interp run")
     ap.add_argument("--cpu-views", type=int, default=4, help="forward views in the CPU sample")
     ap.add_argument("--cpu-slices", type=int, default=16, help="z slices in the CPU backprojection sample")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -110,7 +111,7 @@ def main(argv=None):
     R = n * Hd * Wd
     V = D * H * W
 
-    def run(storage):
+    def run(storage, hw=False):
         v = vol.to(storage)
         times_f, times_b = [], []
         sino = None
@@ -119,16 +120,17 @@ def main(argv=None):
             e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
             torch.cuda.synchronize()
             e0.record()
-            sino = ops.cone_projection3d(v, [n, Hd, Wd], vo, vs, Pt, st, mu, False)
+            sino = ops.cone_projection3d(v, [n, Hd, Wd], vo, vs, Pt, st, mu, hw)
             e1.record()
             s_in = sino if storage == torch.float32 else sino.to(storage)
             e1b = torch.cuda.Event(enable_timing=True)
             e1b.record()
-            rec = ops.cone_backprojection3d(s_in, [D, H, W], vo, vs, Pt, st, mu, False)
+            rec = ops.cone_backprojection3d(s_in, [D, H, W], vo, vs, Pt, st, mu, hw)
             e2.record()
             torch.cuda.synchronize()
             tf, tb = e0.elapsed_time(e1) / 1e3, e1b.elapsed_time(e2) / 1e3
-            log(f"{'step' if timed else 'warmup'} {it} {str(storage)}: fwd {tf * 1e3:.1f} ms, bp {tb * 1e3:.1f} ms")
+            log(f"{'step' if timed else 'warmup'} {it} {str(storage)}{' hw' if hw else ''}: "
+                f"fwd {tf * 1e3:.1f} ms, bp {tb * 1e3:.1f} ms")
             if timed:
                 times_f.append(tf)
                 times_b.append(tb)
@@ -220,6 +222,19 @@ def main(argv=None):
         }
     else:
         del sino
+
+    if not args.no_hw:
+        # hardware_interp: texture clip box [-0.5, N-0.5] and sample positions, exact weights
+        torch.cuda.empty_cache()
+        S_hw = metrics.cone_forward_sample_count(P, cfg["shape"], sp, og, cfg["det"], cfg["step"],
+                                                 hardware_interp=True, device=dev)
+        tfh, tbh, sino_h = run(torch.float32, hw=True)
+        del sino_h
+        out["hardware_interp"] = {
+            "value": round((S_hw + U_bp) / (tfh + tbh) / 1e9, 3), "unit": "Gsamples/s",
+            "t_fwd_ms": round(tfh * 1e3, 2), "t_bp_ms": round(tbh * 1e3, 2), "S_fwd": S_hw,
+            "storage": "f32",
+        }
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(vol, P, sp, og, cfg, args)
