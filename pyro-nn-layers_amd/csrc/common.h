@@ -50,8 +50,12 @@ __device__ __forceinline__ float lerp_ref(float a, float b, float t) {
   return fmaf(b, t, fmaf(a, -t, a));
 }
 
+// Clamp into the zero border [-1, n] with one v_med3_f32.  For every input that is not
+// a signaling NaN this equals fminf(fmaxf(c, -1), n) (a quiet NaN gives -1, as the
+// oracle's std::fmin(std::fmax()) does); arithmetic never produces a signaling NaN.
+// The test suite checks the equality on the device for all 2^32 inputs.
 __device__ __forceinline__ float clamp_apron(float c, float n) {
-  return fminf(fmaxf(c, -1.0f), n);
+  return __builtin_amdgcn_fmed3f(c, -1.0f, n);
 }
 
 // 1.0f / w, correctly rounded, in about half the instructions of the generic IEEE
@@ -61,15 +65,17 @@ __device__ __forceinline__ float clamp_apron(float c, float n) {
 // correction step).  Everything else (zero, infinities, NaN, extreme exponents) takes
 // the IEEE division.  Bit equality with 1.0f / w over all 2^32 inputs is checked on
 // the device by the test suite.
-__device__ __forceinline__ float rcp_rn(float w) {
+__device__ __forceinline__ bool rcp_fast_ok(float w) {
   const float a = fabsf(w);
-  if (a >= 0x1p-125f && a <= 0x1p125f) {
-    float r = __builtin_amdgcn_rcpf(w);
-    r = fmaf(fmaf(-w, r, 1.0f), r, r);
-    return fmaf(fmaf(-w, r, 1.0f), r, r);
-  }
-  return 1.0f / w;
+  return a >= 0x1p-125f && a <= 0x1p125f;
 }
+// Requires rcp_fast_ok(w).
+__device__ __forceinline__ float rcp_fast(float w) {
+  float r = __builtin_amdgcn_rcpf(w);
+  r = fmaf(fmaf(-w, r, 1.0f), r, r);
+  return fmaf(fmaf(-w, r, 1.0f), r, r);
+}
+__device__ __forceinline__ float rcp_rn(float w) { return rcp_fast_ok(w) ? rcp_fast(w) : 1.0f / w; }
 
 // Iteration cap of every ray march (the reference has none and would spin
 // forever when the step is below the float resolution of alpha).

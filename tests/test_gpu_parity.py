@@ -80,6 +80,51 @@ def test_cone_backward_parity(cone, hw):
     assert exact > 0.99
 
 
+def _ray_dirs_f32(inv, Hd, Wd):
+    """The forward kernel's per-ray direction, (u, w, 1) through inv_AR with the same
+    FMA nesting, in float32 (float64 holds each product exactly)."""
+    m = inv.astype(np.float64).reshape(-1, 9)
+    w, u = np.meshgrid(np.arange(Hd, dtype=np.float64), np.arange(Wd, dtype=np.float64), indexing="ij")
+    f = lambda a, b, c: (a * b + c).astype(np.float32).astype(np.float64)
+    return np.stack([f(u[None], m[:, 3 * i, None, None], f(w[None], m[:, 3 * i + 1, None, None],
+                                                            m[:, 3 * i + 2, None, None])) for i in range(3)], -1)
+
+
+@pytest.mark.parametrize("case", ["axis_rays", "far_source", "tiny_step"])
+@pytest.mark.parametrize("hw", [False, True])
+def test_cone_forward_guarded_march(case, hw):
+    """The forward marches without the coordinate clamp and the iteration cap only in
+    waves whose every ray provably needs neither (all direction components non-zero,
+    |source| < 2^18 and |alpha| < 2^19 voxels, step >= 2^-20 |alpha|).  These geometries
+    break one condition each, so the guarded march runs; it must still match the oracle
+    bit for bit."""
+    shape = (12, 10, 14)
+    sp = np.ones(3, np.float32)
+    og = geo.centred_origin(shape, sp).astype(np.float32)
+    rng = np.random.default_rng(8)
+    vol = rng.uniform(0.0, 1.0, shape).astype(np.float32)
+    step = 0.5
+    if case == "axis_rays":      # one view at angle 0, odd detector: a row and a column of rays
+        det = (15, 17)           # have an exactly zero direction component
+        P = geo.circular_projection_matrices(1, 512.0, 1024.0, det, (1.0, 1.0)).astype(np.float32)
+    elif case == "far_source":   # source beyond 2^18 voxels
+        det = (12, 14)
+        P = geo.circular_projection_matrices(4, 4.0e5, 4.0e5 + 200.0, det, (1.0, 1.0), start=0.3).astype(np.float32)
+    else:                        # step below 2^-20 of the ray parameter
+        det = (6, 8)
+        P = geo.circular_projection_matrices(2, 100.0, 200.0, det, (2.5, 2.5), start=0.2).astype(np.float32)
+        step = 2.0e-5
+    inv, src, _, _, _ = orc.cone_setup(P, sp, og)
+    if case == "axis_rays":
+        assert (_ray_dirs_f32(inv, *det) == 0).any()
+    elif case == "far_source":
+        assert np.abs(src).max() >= 2.0 ** 18
+    g = gh.cone_forward(vol, P, sp, og, det, step, hw=hw)[0]
+    r, _, ns = orc.cone_forward(vol, inv, src, det, sp, step, hw=hw)
+    assert ns > 0 and np.abs(r).max() > 0
+    np.testing.assert_array_equal(g, r)
+
+
 def test_cone_hw_mode_within_texture_weight_bound(cone):
     """hardware_interp against a texture unit's fixed-point filter weights: the HIP result
     (exact weights, HW clip box and sample positions) differs from the oracle with weights
