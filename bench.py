@@ -192,8 +192,9 @@ def main(argv=None):
             "traffic_source": "profiles/cone_m1_f32_traffic.json (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
             # measured HBM bytes over the live duration: the share of HBM bandwidth actually used
             "traffic_frac": round(traffic / t_dom / HBM_PEAK, 4) if traffic else None,
-            "limiter": ("vector-memory address path (TA 98% busy, VALU 73%; profiles/cone_m1_f32_counters.md): "
-                        "algorithmic bytes exceed HBM peak because the L1/L2 serve ~90% of corner reads")
+            "limiter": ("vector-memory address path (texture-address unit 98.6% busy, one L1 access per sample; "
+                        "profiles/cone_m1_counters.md): algorithmic bytes exceed HBM peak because the L1 serves "
+                        "~92% of corner reads")
             if dominant == "cone_fwd_kernel" else None,
         },
         "fwd_back": {
@@ -265,7 +266,7 @@ def serve_bench(args):
     from pyronn_layers import geometry as geo
     from pyronn_layers import phantoms as ph
     from pyronn_layers.fdk import ConeFDK, FanFBP
-    from pyronn_layers.serving import GpuExecutor, Scheduler, latency_report, poisson_arrivals
+    from pyronn_layers.serving import GpuExecutor, Request, Scheduler, latency_report, poisson_arrivals
 
     pl.load()
     dev = torch.device("cuda:0")
@@ -335,6 +336,14 @@ def serve_bench(args):
         rep = latency_report(res)
         log(f"serving {name}: {rep['req_per_s']:.2f} req/s p50 {rep['p50_ms']:.1f} ms p99 {rep['p99_ms']:.1f} ms")
         out[name] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in rep.items()}
+        # capacity: the same requests, all queued at t = 0 (throughput limited by the GPU, not arrivals)
+        sat = [Request(r.rid, r.kind, 0.0, r.payload) for r in reqs]
+        res_s = Scheduler(ex, slots=args.serve_slots, hbm_budget=160 << 30).run(sat, idle=lambda dt: time.sleep(
+            dt) if dt > 0 else None)
+        rep_s = latency_report(res_s)
+        log(f"serving {name} saturated: {rep_s['req_per_s']:.2f} req/s")
+        out[name]["capacity"] = {k: round(rep_s[k], 3) for k in ("req_per_s", "span_s")}
+        out[name]["capacity"]["completed"] = rep_s["completed"]
         del ex, plans, payloads
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
