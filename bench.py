@@ -65,6 +65,8 @@ def main(argv=None):
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-batch run")
     ap.add_argument("--batched-only", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the C1-C3 configuration measurements")
+    ap.add_argument("--configs-only", action="store_true")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path.",
@@ -143,6 +145,9 @@ def main(argv=None):
         return 0
     if args.batched_only:
         print(json.dumps(batched_bench(args)), flush=True)
+        return 0
+    if args.configs_only:
+        print(json.dumps(configs_bench(args)), flush=True)
         return 0
     if args.bf16_only:
         tf, tb, _ = run(torch.bfloat16)
@@ -247,6 +252,9 @@ def main(argv=None):
     if not args.no_batched:
         torch.cuda.empty_cache()
         out["uneven_batch"] = batched_bench(args)
+    if not args.no_configs:
+        torch.cuda.empty_cache()
+        out["configs"] = configs_bench(args)
 
     print(json.dumps(out), flush=True)
     return 0
@@ -348,6 +356,139 @@ def serve_bench(args):
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
     return out
+
+
+def _time_events(fn, reps):
+    """Mean duration (s) of fn() over reps launches, HIP events on the launch stream."""
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / 1e3 / reps
+
+
+def configs_bench(args):
+    """The other benchmark configurations, each as forward + backprojection of its
+    sinogram, FP32, with the same units as the headline line:
+      C1 parallel 256^2 (1 mm), 180 views over pi, 363 detector pixels, step 0.2 (the
+         reference's fixed factor), 12 seeded ellipses;
+      C2 fan 512^2 (0.5 mm), 360 views over 2 pi, 725 pixels, SID 1000, SDD 1200, step 0.5;
+      C3 cone 256^3 (1 mm), 360 views over 2 pi, 400 x 400 (1 mm), SID 750, SDD 1200,
+         step 0.5, software and hardware_interp.
+    Gsamples/s = (forward samples + backprojection updates) / (t_fwd + t_bp); the 2D ops
+    are also timed on batches of 32 images (throughput regime).  roofline_frac uses the
+    algorithmic bytes (2D: 4 neighbours per forward sample, 2 per backprojection update;
+    3D: 8 and 4) over 8 TB/s.  The CPU oracle is timed on the same problems (C1, C2 in full;
+    C3 on 8 evenly spaced views and 8 central slices)."""
+    import numpy as np
+    import torch
+
+    import oracle as orc
+    import pyronn_layers as pl
+    from pyronn_layers import geometry as geo
+    from pyronn_layers import metrics
+    from pyronn_layers import phantoms as ph
+
+    pl.load()
+    dev = torch.device("cuda:0")
+    ops = torch.ops.pyronn_layers
+    res = {}
+    t1 = lambda v: torch.full((1, 1), float(v), device=dev)
+
+    def twod(name, fan, H, W, s, n, arc, det, ds, sid, sdd, step, n_ell, seed):
+        org = geo.centred_origin((H, W), s)
+        e = ph.random_ellipses(seed, n_ell, (H, W), s)
+        img = ph.rasterise_ellipses(e, (H, W), s, org).astype(np.float32)
+        rays = (geo.fan_central_ray_vectors(n, arc) if fan else geo.parallel_ray_vectors(n, arc)).astype(np.float32)
+        do = -(det - 1) / 2 * ds
+        fwd = ops.fan_projection2d if fan else ops.parallel_projection2d
+        bwd = ops.fan_backprojection2d if fan else ops.parallel_backprojection2d
+        out = {}
+        for B in (1, 32):
+            im = torch.tensor(img, device=dev)[None].repeat(B, 1, 1).contiguous()
+            g = [torch.tensor(org[None], dtype=torch.float32, device=dev).repeat(B, 1), t1(do).repeat(B, 1),
+                 torch.tensor(np.asarray(s, np.float32)[None], device=dev).repeat(B, 1), t1(ds).repeat(B, 1)]
+            if fan:
+                g += [t1(sid).repeat(B, 1), t1(sdd).repeat(B, 1)]
+            r = torch.tensor(rays[None], device=dev).repeat(B, 1, 1).contiguous()
+            sino = fwd(im, [n, det], *g, r, step)
+            tf = _time_events(lambda: fwd(im, [n, det], *g, r, step), 20)
+            tb = _time_events(lambda: bwd(sino, [H, W], *g, r), 20)
+            out[B] = (tf, tb)
+        t0 = time.perf_counter()
+        f = orc.fan_forward if fan else orc.parallel_forward
+        b = orc.fan_backward if fan else orc.parallel_backward
+        kw = dict(sid=sid, sdd=sdd) if fan else {}
+        sref, _, S = f(img, rays, det, org, s, do, ds, step_factor=step, **kw)
+        cf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        b(sref, rays, (H, W), org, s, do, ds, **kw)
+        cb = time.perf_counter() - t0
+        U = H * W * n
+        Bytes = S * 4 * 4 + n * det * 4 + U * 2 * 4 + H * W * 4
+        tf, tb = out[1]
+        tf32, tb32 = out[32]
+        res[name] = {
+            "t_fwd_ms": round(tf * 1e3, 4), "t_bp_ms": round(tb * 1e3, 4), "S_fwd": int(S), "U_bp": U,
+            "gsamples_s": round((S + U) / (tf + tb) / 1e9, 3),
+            "gsamples_s_batch32": round(32 * (S + U) / (tf32 + tb32) / 1e9, 3),
+            "roofline_frac_batch32": round(32 * Bytes / (tf32 + tb32) / HBM_PEAK, 4),
+            "cpu_oracle_gsamples_s": round((S + U) / (cf + cb) / 1e9, 4), "cpu_threads": orc.num_threads(),
+        }
+        log(f"configs {name}: GPU {res[name]['gsamples_s']} Gsamples/s (batch 32: {res[name]['gsamples_s_batch32']}), "
+            f"CPU {res[name]['cpu_oracle_gsamples_s']}")
+
+    log("configs: C1 parallel")
+    twod("C1_parallel_256", False, 256, 256, (1.0, 1.0), 180, math.pi, 363, 1.0, None, None, 0.2, 12, 1)
+    log("configs: C2 fan")
+    twod("C2_fan_512", True, 512, 512, (0.5, 0.5), 360, 2 * math.pi, 725, 1.0, 1000.0, 1200.0, 0.5, 16, 2)
+
+    log("configs: C3 cone")
+    shape, det, n = (256, 256, 256), (400, 400), 360
+    sp = np.ones(3, np.float32)
+    og = geo.centred_origin(shape, sp).astype(np.float32)
+    P = geo.circular_projection_matrices(n, 750.0, 1200.0, det, (1.0, 1.0)).astype(np.float32)
+    e = ph.random_ellipsoids(3, 20, shape, sp, axes_range_vox=(8.0, 100.0))
+    vol = ph.rasterise_ellipsoids_torch(e, shape, sp, og, device=dev)[None].contiguous()
+    geo_t = (torch.tensor(og[None], device=dev), torch.tensor(sp[None], device=dev), torch.tensor(P[None], device=dev),
+             t1(0.5), t1(1.0))
+    U = int(np.prod(shape)) * n
+    for hw in (False, True):
+        S = metrics.cone_forward_sample_count(P, shape, sp, og, det, 0.5, hardware_interp=hw, device=dev)
+        sino = ops.cone_projection3d(vol, [n, *det], *geo_t, hw)
+        tf = _time_events(lambda: ops.cone_projection3d(vol, [n, *det], *geo_t, hw), 5)
+        tb = _time_events(lambda: ops.cone_backprojection3d(sino, list(shape), *geo_t, hw), 5)
+        Bytes = S * 8 * 4 + n * det[0] * det[1] * 4 + U * 4 * 4 + int(np.prod(shape)) * 4
+        res["C3_cone_256" + ("_hw" if hw else "")] = {
+            "t_fwd_ms": round(tf * 1e3, 3), "t_bp_ms": round(tb * 1e3, 3), "S_fwd": int(S), "U_bp": U,
+            "gsamples_s": round((S + U) / (tf + tb) / 1e9, 3),
+            "roofline_frac": round(Bytes / (tf + tb) / HBM_PEAK, 4),
+        }
+        log(f"configs C3 hw={hw}: fwd {tf * 1e3:.2f} ms bp {tb * 1e3:.2f} ms")
+    # CPU oracle on a bounded sample of C3 (software interpolation)
+    vh = vol[0].cpu().numpy()
+    inv, src, _, _, _ = orc.cone_setup(P, sp, og)
+    views = np.linspace(0, n, 8, endpoint=False).astype(int)
+    t0 = time.perf_counter()
+    Sc = 0
+    for v in views:
+        _, _, ns = orc.cone_forward(vh, inv[v:v + 1], src[v:v + 1], det, sp, 0.5)
+        Sc += ns
+    cf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    orc.cone_backward(ops.cone_projection3d(vol, [n, *det], *geo_t, False)[0].cpu().numpy(), P, shape, sp, og,
+                      zrange=(124, 132))
+    cb = time.perf_counter() - t0
+    res["C3_cone_256"]["cpu_oracle_gsamples_s"] = round((Sc + 8 * 256 * 256 * n) / (cf + cb) / 1e9, 4)
+    res["C3_cone_256"]["cpu_sample"] = "8 of 360 forward views + 8 of 256 backprojection slices"
+    return res
 
 
 def batched_bench(args):

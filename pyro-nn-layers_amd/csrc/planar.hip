@@ -19,22 +19,49 @@ constexpr float kPi = 3.14159265359f;  // par_backprojector_2D_CudaKernel.cu.cc:
 __device__ __forceinline__ float i2p(float i, float o, float s) { return fmaf(i, s, o); }
 __device__ __forceinline__ float p2i(float p, float o, float s) { return (p - o) / s; }
 
+// 1 / s when s is a positive power of two well inside the normal range, else 0.  For
+// such s, x / s and x * (1 / s) are the correctly rounded value of the same real number,
+// so they are equal for every x; the kernels then multiply instead of dividing.
+__device__ __forceinline__ float pow2_reciprocal(float s) {
+  const unsigned b = __float_as_uint(s);
+  const unsigned e = (b >> 23) & 0xffu;
+  return ((b & 0x807fffffu) == 0u && e > 1u && e < 253u) ? __uint_as_float((254u - e) << 23) : 0.0f;
+}
+// physical_to_index with the division replaced by the exact product when POW2
+template <bool POW2>
+__device__ __forceinline__ float p2i_k(float p, float o, float s, float inv) {
+  return POW2 ? (p - o) * inv : (p - o) / s;
+}
+
+// The four neighbours a lane read last (consecutive march samples usually share them).
+struct Tex2dRegs {
+  int x0 = 0x7fffffff, y0 = 0;
+  float p00 = 0.f, p01 = 0.f, p10 = 0.f, p11 = 0.f;
+};
+
+// tex2d with the neighbours kept in registers across calls: loads only when (x0, y0)
+// changes; the same values and the same blend as tex2d.
 template <typename TS>
-__device__ __forceinline__ float tex2d(const TS *__restrict__ img, int W, int H, float cx,
-                                       float cy) {
+__device__ __forceinline__ float tex2d_cached(const TS *__restrict__ img, int W, int H, float cx,
+                                              float cy, Tex2dRegs &r) {
   float bx = cx - 0.5f, by = cy - 0.5f;
   if (!(bx > -2.0f && bx < (float)(W + 1) && by > -2.0f && by < (float)(H + 1))) return 0.0f;
   const float fx = floorf(bx), fy = floorf(by);
-  const int x0 = (int)fx, y0 = (int)fy, x1 = x0 + 1, y1 = y0 + 1;
-  const bool vx0 = x0 >= 0 && x0 < W, vx1 = x1 >= 0 && x1 < W;
-  const bool vy0 = y0 >= 0 && y0 < H, vy1 = y1 >= 0 && y1 < H;
-  const float p00 = (vx0 && vy0) ? to_f32(img[y0 * W + x0]) : 0.0f;
-  const float p01 = (vx1 && vy0) ? to_f32(img[y0 * W + x1]) : 0.0f;
-  const float p10 = (vx0 && vy1) ? to_f32(img[y1 * W + x0]) : 0.0f;
-  const float p11 = (vx1 && vy1) ? to_f32(img[y1 * W + x1]) : 0.0f;
+  const int x0 = (int)fx, y0 = (int)fy;
+  if (x0 != r.x0 || y0 != r.y0) {
+    r.x0 = x0;
+    r.y0 = y0;
+    const int x1 = x0 + 1, y1 = y0 + 1;
+    const bool vx0 = x0 >= 0 && x0 < W, vx1 = x1 >= 0 && x1 < W;
+    const bool vy0 = y0 >= 0 && y0 < H, vy1 = y1 >= 0 && y1 < H;
+    r.p00 = (vx0 && vy0) ? to_f32(img[y0 * W + x0]) : 0.0f;
+    r.p01 = (vx1 && vy0) ? to_f32(img[y0 * W + x1]) : 0.0f;
+    r.p10 = (vx0 && vy1) ? to_f32(img[y1 * W + x0]) : 0.0f;
+    r.p11 = (vx1 && vy1) ? to_f32(img[y1 * W + x1]) : 0.0f;
+  }
   const float xd = bx - fx, yd = by - fy;
-  const float p0 = lerp_ref(p00, p01, xd);
-  const float p1 = lerp_ref(p10, p11, xd);
+  const float p0 = lerp_ref(r.p00, r.p01, xd);
+  const float p1 = lerp_ref(r.p10, r.p11, xd);
   return lerp_ref(p0, p1, yd);
 }
 
@@ -52,9 +79,10 @@ __device__ __forceinline__ float tex1d(const TS *__restrict__ row, int W, float 
 // kernel_project2D, par_projector_2D_CudaKernel.cu.cc:28-102 (same code in
 // fan_projector_2D_CudaKernel.cu.cc:27-102): march in mm, box edges with the
 // -0.5 mm offset, first sample exactly at the entry point, tail for a positive sum.
-template <typename TS>
+template <typename TS, bool POW2>
 __device__ float march2d(const TS *__restrict__ img, int W, int H, float ox, float oy, float sx,
-                         float sy, float s0, float s1, float r0, float r1, float step) {
+                         float sy, float ix, float iy, float s0, float s1, float r0, float r1,
+                         float step) {
   if (!(step > 0.0f)) return __builtin_nanf("");
   float amin = 0.0f, amax = __builtin_inff();
   if (0.0f != r0) {
@@ -72,8 +100,10 @@ __device__ float march2d(const TS *__restrict__ img, int W, int H, float ox, flo
     amax = fminf(amax, fmaxf(a0, a1));
   }
   float pix = 0.0f, px = 0.0f, py = 0.0f;
+  Tex2dRegs regs;
   const int cap = march_cap(amin, amax, step);
-#define PYRONN_TEX2D(X, Y) tex2d(img, W, H, p2i((X), ox, sx) + 0.5f, p2i((Y), oy, sy) + 0.5f)
+#define PYRONN_TEX2D(X, Y) \
+  tex2d_cached(img, W, H, p2i_k<POW2>((X), ox, sx, ix) + 0.5f, p2i_k<POW2>((Y), oy, sy, iy) + 0.5f, regs)
   if (amin < amax) {
     px = fmaf(amin, r0, s0);
     py = fmaf(amin, r1, s1);
@@ -141,7 +171,11 @@ proj2d_kernel(const TS *__restrict__ vol, float *__restrict__ sino, const float 
   }
   const float step = step_factor * fminf(sx, sy);
   const TS *img = vol + (size_t)b * H * W;
-  float pix = march2d(img, W, H, ox, oy, sx, sy, s0, s1, r0, r1, step);
+  // spacings are per batch item, so the choice is uniform over the block
+  const float ix = pow2_reciprocal(sx), iy = pow2_reciprocal(sy);
+  const float pix = (ix != 0.0f && iy != 0.0f)
+                        ? march2d<TS, true>(img, W, H, ox, oy, sx, sy, ix, iy, s0, s1, r0, r1, step)
+                        : march2d<TS, false>(img, W, H, ox, oy, sx, sy, ix, iy, s0, s1, r0, r1, step);
   const float a = r0 * sx, c = r1 * sy;
   sino[vb * det + d] = pix * sqrtf(fmaf(c, c, a * a));
 }
@@ -159,6 +193,38 @@ __device__ __forceinline__ void isect2(float p1x, float p1y, float p2x, float p2
   const float g = fmaf(-p3y, p4x, p3x * p4y);
   ox = fmaf(-a, g, e * d) / den;
   oy = fmaf(-c, g, e * b) / den;
+}
+
+// Sum over all views for the pixel at physical (pcx, pcy).
+template <typename TS, bool FAN, bool POW2>
+__device__ __forceinline__ float back2d_sum(const TS *__restrict__ sb, const float *__restrict__ rb, int n,
+                                            int det, float pcx, float pcy, float do_, float ds,
+                                            float ids, float sid, float sdd) {
+  float val = 0.0f;
+  if (!FAN) {
+    for (int v = 0; v < n; ++v) {
+      const float c0 = rb[v * 2 + 0], c1 = rb[v * 2 + 1];
+      const float s = fmaf(pcy, c0, pcx * (-c1));
+      const float si = p2i_k<POW2>(s, do_, ds, ids);
+      val += tex1d(sb + (size_t)v * det, det, si + 0.5f);
+    }
+  } else {
+    for (int v = 0; v < n; ++v) {
+      const float c0 = rb[v * 2 + 0], c1 = rb[v * 2 + 1];
+      const float d0 = -c1, d1 = c0;
+      const float srx = c0 * (-sid), sry = c1 * (-sid);
+      const float cpx = fmaf(c0, sdd, srx), cpy = fmaf(c1, sdd, sry);
+      float ix, iy;
+      isect2(pcx, pcy, srx, sry, cpx, cpy, cpx + d0, cpy + d1, ix, iy);
+      const float ex = pcx - srx, ey = pcy - sry;
+      const float dw = 1.0f / sqrtf(fmaf(ey, ey, ex * ex));
+      const float s = fmaf(iy, d1, ix * d0);
+      const float si = p2i_k<POW2>(s, do_, ds, ids);
+      const float f = tex1d(sb + (size_t)v * det, det, si + 0.5f);
+      val = fmaf(f * dw, dw, val);
+    }
+  }
+  return val;
 }
 
 // One lane per pixel; loop over all views.
@@ -181,31 +247,18 @@ back2d_kernel(const TS *__restrict__ sino, float *__restrict__ vol, const float 
   const float pcx = i2p((float)x, ox, sx), pcy = i2p((float)y, oy, sy);
   const float *rb = rays + (size_t)b * n * 2;
   const TS *sb = sino + (size_t)b * n * det;
-  float val = 0.0f;
+  // the detector spacing is per batch item, so the choice is uniform over the block
+  const float ids = pow2_reciprocal(ds);
+  float sid = 0.0f, sdd = 0.0f;
+  if (FAN) {  // the parallel op passes no source distances
+    sid = sid_p[b];
+    sdd = sdd_p[b];
+  }
+  const float val = ids != 0.0f ? back2d_sum<TS, FAN, true>(sb, rb, n, det, pcx, pcy, do_, ds, ids, sid, sdd)
+                                : back2d_sum<TS, FAN, false>(sb, rb, n, det, pcx, pcy, do_, ds, ids, sid, sdd);
   if (!FAN) {
-    for (int v = 0; v < n; ++v) {
-      const float c0 = rb[v * 2 + 0], c1 = rb[v * 2 + 1];
-      const float s = fmaf(pcy, c0, pcx * (-c1));
-      const float si = p2i(s, do_, ds);
-      val += tex1d(sb + (size_t)v * det, det, si + 0.5f);
-    }
     vol[((size_t)b * H + y) * W + x] = (kPi * val) / (float)n;
   } else {
-    const float sid = sid_p[b], sdd = sdd_p[b];
-    for (int v = 0; v < n; ++v) {
-      const float c0 = rb[v * 2 + 0], c1 = rb[v * 2 + 1];
-      const float d0 = -c1, d1 = c0;
-      const float srx = c0 * (-sid), sry = c1 * (-sid);
-      const float cpx = fmaf(c0, sdd, srx), cpy = fmaf(c1, sdd, sry);
-      float ix, iy;
-      isect2(pcx, pcy, srx, sry, cpx, cpy, cpx + d0, cpy + d1, ix, iy);
-      const float ex = pcx - srx, ey = pcy - sry;
-      const float dw = 1.0f / sqrtf(fmaf(ey, ey, ex * ex));
-      const float s = fmaf(iy, d1, ix * d0);
-      const float si = p2i(s, do_, ds);
-      const float f = tex1d(sb + (size_t)v * det, det, si + 0.5f);
-      val = fmaf(f * dw, dw, val);
-    }
     vol[((size_t)b * H + y) * W + x] = (((sid * sdd) * kPi) * val) / (float)n;
   }
 }

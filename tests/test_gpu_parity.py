@@ -327,3 +327,41 @@ def test_2d_batch_items_match_single_calls(par):
     for b in range(2):
         one = gh.forward2d(False, imgs[b], rays[b], par["det"], par["org"], par["s"], par["do"], par["ds"])
         np.testing.assert_array_equal(both[b], one[0])
+
+
+@pytest.mark.parametrize("fan_beam", [False, True])
+def test_2d_non_power_of_two_spacings_and_mixed_batch(fan_beam):
+    """The 2D kernels multiply by 1/s instead of dividing when a spacing is a power of two
+    (then both give the same bits).  Spacings 0.9 x 1.1 mm and 0.75 mm detector pixels take
+    the division path; a batch whose second item has power-of-two spacings mixes both
+    paths.  Every item must match the oracle and its single-item call."""
+    H, W = 96, 112
+    spacings = [np.array([0.9, 1.1], np.float32), np.array([0.5, 0.5], np.float32)]
+    dss = [np.float32(0.75), np.float32(1.0)]
+    n, det = 72, 181
+    rays = (geo.fan_central_ray_vectors(n, 2 * math.pi) if fan_beam else geo.parallel_ray_vectors(n)).astype(
+        np.float32)
+    kw = dict(sid=np.float32(600.0), sdd=np.float32(900.0)) if fan_beam else {}
+    imgs, orgs, dos = [], [], []
+    for k, s in enumerate(spacings):
+        org = geo.centred_origin((H, W), s).astype(np.float32)
+        e = ph.random_ellipses(30 + k, 10, (H, W), s, axes_range_px=(4, 40))
+        imgs.append(ph.rasterise_ellipses(e, (H, W), s, org))
+        orgs.append(org)
+        dos.append(np.float32(-(det - 1) / 2 * dss[k]))
+    fwd_o = orc.fan_forward if fan_beam else orc.parallel_forward
+    bwd_o = orc.fan_backward if fan_beam else orc.parallel_backward
+    okw = dict(sid=kw["sid"], sdd=kw["sdd"]) if fan_beam else {}
+    both = gh.forward2d(fan_beam, np.stack(imgs), rays, det, np.stack(orgs), np.stack(spacings), np.stack(dos),
+                        np.stack(dss), step_factor=0.5, **kw)
+    for k in range(2):
+        one = gh.forward2d(fan_beam, imgs[k], rays, det, orgs[k], spacings[k], dos[k], dss[k], step_factor=0.5,
+                           **kw)[0]
+        np.testing.assert_array_equal(both[k], one)
+        r, _, _ = fwd_o(imgs[k], rays, det, orgs[k], spacings[k], dos[k], dss[k], step_factor=0.5, **okw)
+        l2, mx, exact = gh.parity(one, r)
+        assert exact == 1.0, (k, exact)
+        gb = gh.backward2d(fan_beam, r, rays, (H, W), orgs[k], spacings[k], dos[k], dss[k], **kw)[0]
+        rb, _ = bwd_o(r, rays, (H, W), orgs[k], spacings[k], dos[k], dss[k], **okw)
+        l2, mx, exact = gh.parity(gb, rb)
+        assert exact == 1.0, (k, exact)
