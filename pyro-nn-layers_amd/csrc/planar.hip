@@ -217,7 +217,8 @@ __device__ __forceinline__ float back2d_sum(const TS *__restrict__ sb, const flo
       float ix, iy;
       isect2(pcx, pcy, srx, sry, cpx, cpy, cpx + d0, cpy + d1, ix, iy);
       const float ex = pcx - srx, ey = pcy - sry;
-      const float dw = 1.0f / sqrtf(fmaf(ey, ey, ex * ex));
+      // 1 / |x - source| as the reference's division (rcp_rn equals 1.0f / w for every w)
+      const float dw = rcp_rn(sqrtf(fmaf(ey, ey, ex * ex)));
       const float s = fmaf(iy, d1, ix * d0);
       const float si = p2i_k<POW2>(s, do_, ds, ids);
       const float f = tex1d(sb + (size_t)v * det, det, si + 0.5f);
