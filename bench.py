@@ -62,6 +62,8 @@ def main(argv=None):
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--serve-chunks", type=int, default=8,
+                    help="view ranges of a 512^3 request uploaded while earlier ranges reconstruct (1 = off)")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-batch run")
     ap.add_argument("--batched-only", action="store_true")
@@ -321,7 +323,8 @@ def serve_bench(args):
                        "256^3/360 views 400x400, 20%% cone 512^3/720 views 1024x768; noisy sinograms "
                        "(sigma = 0.01 max) from pinned host memory; latency = arrival -> volume in host memory"
                        % (args.serve_rate, args.serve_requests),
-           "slots": args.serve_slots, "hip_graphs": True}
+           "slots": args.serve_slots, "hip_graphs": True,
+           "cone512_view_ranges": args.serve_chunks if args.serve_chunks > 1 else None}
     for name, storage in (("f32", torch.float32), ("bf16", torch.bfloat16)):
         payloads = {}
         for k, s_ in clean.items():
@@ -332,7 +335,8 @@ def serve_bench(args):
                 payloads[k].append(noisy.to(storage).cpu().pin_memory())
         plans = {k: (FanFBP(g, storage=storage) if k.startswith("fan") else ConeFDK(g, storage=storage))
                  for k, g in geoms.items()}
-        ex = GpuExecutor(plans, slots=args.serve_slots, use_graphs=True)
+        chunked = {"cone512": args.serve_chunks} if args.serve_chunks > 1 else {}
+        ex = GpuExecutor(plans, slots=args.serve_slots, use_graphs=True, chunked=chunked)
         log(f"serving {name}: capturing graphs")
         ex.warm()
         reqs = poisson_arrivals(5, args.serve_rate, args.serve_requests, mix)

@@ -120,6 +120,28 @@ int pyronn_cone_backprojection3d(const void *sinogram, int sinogram_dtype, int b
                                  size_t workspace_bytes, pyronn_stream_t stream);
 
 /*
+ * Backprojection of one range of views into a running sum.  No counterpart in the
+ * reference: it serves callers that receive a sinogram in view chunks (a server that
+ * overlaps the upload of chunk k + 1 with the backprojection of chunk k).  sinogram and
+ * projection_matrices hold the chunk's n_views views; the workspace is sized by
+ * pyronn_cone_backprojection3d_workspace_size for the chunk.  mode bit 0: continue the
+ * per-voxel running sums stored in volume by the previous chunk (else start from zero);
+ * bit 1: multiply by projection_multiplier when done (the last chunk).  Chunks of one
+ * sinogram processed in view order with modes 0, 1, ..., 1, 3 give bit for bit the
+ * result of pyronn_cone_backprojection3d over all views (mode 2 alone is that operator):
+ * every voxel accumulates its views in the same order and the running sum between chunks
+ * is the same float the one-pass accumulation holds.
+ */
+int pyronn_cone_backprojection3d_views(const void *sinogram, int sinogram_dtype, int batch,
+                                       int n_views, int det_h, int det_w,
+                                       const float *volume_origin, const float *volume_spacing,
+                                       const float *projection_matrices,
+                                       const float *projection_multiplier, int D, int H, int W,
+                                       int hardware_interp, int mode, float *volume,
+                                       void *workspace, size_t workspace_bytes,
+                                       pyronn_stream_t stream);
+
+/*
  * ParallelProjection2D: volume [B][H][W] -> sinogram [B][n][det].
  * step_factor scales min(spacing) to the march step; the reference hard-codes
  * 0.2 (cc/kernels/par_projector_2D_CudaKernel.cu.cc:166), pass 0.2 for parity.

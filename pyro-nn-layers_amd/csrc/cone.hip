@@ -524,7 +524,9 @@ struct BpProblem {
   const float *origin;   // [3] (z, y, x)
   int W, H, D, n, Hd, Wd, tiles_x, tiles_y, zblocks;
   long long begin;
+  int flags;             // kBpContinue: start from the running sums in vol; kBpScale: write sum * mult
 };
+constexpr int kBpContinue = 1, kBpScale = 2;
 
 template <typename TS, bool HW, int ZPT>
 __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int zb) {
@@ -541,7 +543,9 @@ __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int
 #pragma unroll
   for (int q = 0; q < ZPT; ++q) {
     z[q] = fmaf((float)(k0 + q), sz, oz);
-    acc[q] = 0.0f;
+    // a running sum continued from an earlier range of views is the same float the
+    // accumulation would hold here in one pass, so chunked calls reproduce it exactly
+    acc[q] = ((Q.flags & kBpContinue) && k0 + q < D) ? Q.vol[((size_t)(k0 + q) * H + j) * W + i] : 0.0f;
   }
   const int Wc = Wd + 2;
   const size_t img_stride = (size_t)(Hd + 2) * Wc * 4;
@@ -563,10 +567,11 @@ __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int
       acc[q] = fmaf(f * ww, ww, acc[q]);
     }
   }
-  const float mu = Q.mult[0];
+  const bool scale = (Q.flags & kBpScale) != 0;
+  const float mu = scale ? Q.mult[0] : 1.0f;
 #pragma unroll
   for (int q = 0; q < ZPT; ++q) {
-    if (k0 + q < D) Q.vol[((size_t)(k0 + q) * H + j) * W + i] = acc[q] * mu;
+    if (k0 + q < D) Q.vol[((size_t)(k0 + q) * H + j) * W + i] = scale ? acc[q] * mu : acc[q];
   }
 }
 
@@ -679,11 +684,11 @@ static void launch_det_cells(const void *sino, void *cells, int images, int Hd, 
 template <typename TS>
 static void launch_bp(bool hw, const void *cells, float *vol, const float *P, const float *mult,
                       const float *spacing, const float *origin, int B, int D, int H, int W, int n,
-                      int Hd, int Wd, hipStream_t stream) {
+                      int Hd, int Wd, hipStream_t stream, int flags = kBpScale) {
   constexpr int Z = bp_zpt<TS>();
   const int zblocks = (D + Z - 1) / Z;
   BpProblem Q{cells, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd,
-              (W + 15) / 16, (H + 15) / 16, zblocks, 0};
+              (W + 15) / 16, (H + 15) / 16, zblocks, 0, flags};
   dim3 grid((W + 15) / 16, (H + 15) / 16, zblocks * B);
   const size_t cs = det_cells(n, Hd, Wd) * 4;
   if (hw)
@@ -846,6 +851,57 @@ extern "C" int pyronn_cone_backprojection3d(const void *sinogram, int sinogram_d
   return PYRONN_OK;
 }
 
+extern "C" int pyronn_cone_backprojection3d_views(
+    const void *sinogram, int sinogram_dtype, int batch, int n_views, int det_h, int det_w,
+    const float *volume_origin, const float *volume_spacing, const float *projection_matrices,
+    const float *projection_multiplier, int D, int H, int W, int hardware_interp, int mode,
+    float *volume, void *workspace, size_t workspace_bytes, pyronn_stream_t stream) {
+  clear_error();
+  PYRONN_CHECK_ARG(sinogram && volume_origin && volume_spacing && projection_matrices &&
+                       projection_multiplier && volume && workspace,
+                   "cone_backprojection3d_views: null pointer");
+  PYRONN_CHECK_ARG(dtype_ok(sinogram_dtype), "cone_backprojection3d_views: bad sinogram dtype %d",
+                   sinogram_dtype);
+  PYRONN_CHECK_ARG(mode >= 0 && mode <= 3, "cone_backprojection3d_views: bad mode %d", mode);
+  PYRONN_CHECK_ARG(batch > 0 && D > 0 && H > 0 && W > 0 && n_views > 0 && det_h > 0 && det_w > 0,
+                   "cone_backprojection3d_views: bad shape B=%d D=%d H=%d W=%d n=%d det=%dx%d", batch,
+                   D, H, W, n_views, det_h, det_w);
+  PYRONN_CHECK_ARG((long long)((D + 3) / 4) * batch <= 65535,
+                   "cone_backprojection3d_views: batch x depth too large for one launch");
+  if ((size_t)(det_h + 2) * (det_w + 2) >= (size_t)INT32_MAX / 4 || det_w + 2 >= (1 << 23) ||
+      det_h + 2 >= (1 << 23)) {
+    set_error("cone_backprojection3d_views: detector too large");
+    return PYRONN_ERR_TOO_LARGE;
+  }
+  const size_t need =
+      pyronn_cone_backprojection3d_workspace_size(batch, n_views, det_h, det_w, sinogram_dtype);
+  if (workspace_bytes < need) {
+    set_error("cone_backprojection3d_views: workspace %zu bytes < required %zu", workspace_bytes,
+              need);
+    return PYRONN_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const bool hw = hardware_interp != 0;
+  const int images = batch * n_views;
+  const int flags = ((mode & 1) ? kBpContinue : 0) | ((mode & 2) ? kBpScale : 0);
+  if (sinogram_dtype == PYRONN_F32) {
+    launch_det_cells<float>(sinogram, workspace, images, det_h, det_w, s);
+    launch_bp<float>(hw, workspace, volume, projection_matrices, projection_multiplier,
+                     volume_spacing, volume_origin, batch, D, H, W, n_views, det_h, det_w, s, flags);
+  } else if (sinogram_dtype == PYRONN_BF16) {
+    launch_det_cells<bf16s>(sinogram, workspace, images, det_h, det_w, s);
+    launch_bp<bf16s>(hw, workspace, volume, projection_matrices, projection_multiplier,
+                     volume_spacing, volume_origin, batch, D, H, W, n_views, det_h, det_w, s, flags);
+  } else {
+    launch_det_cells<_Float16>(sinogram, workspace, images, det_h, det_w, s);
+    launch_bp<_Float16>(hw, workspace, volume, projection_matrices, projection_multiplier,
+                        volume_spacing, volume_origin, batch, D, H, W, n_views, det_h, det_w, s,
+                        flags);
+  }
+  PYRONN_CHECK_LAUNCH("cone_bp_kernel");
+  return PYRONN_OK;
+}
+
 // =====================================================================================
 // Batched launches over problems of different shapes
 // =====================================================================================
@@ -948,7 +1004,7 @@ static int bp_batched(const pyronn_cone_problem *pr, int count, bool hw, int bpc
     const int tx = (p.W + 15) / 16, ty = (p.H + 15) / 16;
     bb.p[k] = BpProblem{cells, p.output, p.projection_matrices, p.projection_multiplier,
                         p.volume_spacing, p.volume_origin, p.W, p.H, p.D, p.n_views, p.det_h,
-                        p.det_w, tx, ty, zblocks, begin};
+                        p.det_w, tx, ty, zblocks, begin, kBpScale};
     begin += (long long)tx * ty * zblocks;
   }
   bb.count = count;

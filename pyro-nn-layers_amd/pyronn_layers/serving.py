@@ -151,10 +151,16 @@ def latency_report(results: List[Result]) -> Dict[str, Any]:
 class GpuExecutor:
     """Runs FDK / FBP plans on one GPU.  Each slot has its own stream, device input
     buffer, plan workspace, pinned host output buffer and (optionally) one captured
-    graph per geometry class."""
+    graph per geometry class.
+
+    Classes listed in `chunked` (cone FDK plans; value = number of view ranges) are run
+    with ConeFDK.run_chunked instead: the slot's copy stream uploads view range k + 1 while
+    its compute stream filters and backprojects range k, so a large request's upload
+    overlaps its own reconstruction (eager launches; the chunking is fixed per class, so
+    a request's result is the same alone and in any mix)."""
 
     def __init__(self, plans: Dict[str, Any], slots: int, use_graphs: bool = True, keep_outputs: bool = False,
-                 device="cuda"):
+                 device="cuda", chunked: Dict[str, int] | None = None):
         import torch
 
         self.torch = torch
@@ -162,7 +168,9 @@ class GpuExecutor:
         self.device = torch.device(device)
         self.use_graphs = use_graphs
         self.keep = keep_outputs
+        self.chunked = dict(chunked or {})
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(slots)]
+        self.copy_streams = [torch.cuda.Stream(device=self.device) for _ in range(slots)] if self.chunked else []
         self._slot: Dict[tuple, Dict[str, Any]] = {}
         self._fp = {k: self._estimate(k) for k in plans}
 
@@ -181,6 +189,10 @@ class GpuExecutor:
         nin = int(np.prod(self._in_shape(kind)))
         nout = int(np.prod(self._out_shape(kind)))
         e = p.storage.itemsize
+        if kind in self.chunked:
+            # two range buffers + filtered range + filter workspace + detector cells of one range + output
+            nk = -(-nin // self.chunked[kind])
+            return nk * (2 * e + e + 8 + 4 * e) + nout * 4
         # input + filtered + filter workspace (2 x f32) + detector cells (4 x) + output
         return nin * (e + e + 8 + 4 * e) + nout * 4
 
@@ -194,6 +206,17 @@ class GpuExecutor:
             return b
         torch = self.torch
         p = self.plans[kind]
+        if kind in self.chunked:
+            ranges = p.chunk_bounds(self.chunked[kind])
+            kmax = max(hi - lo for lo, hi in ranges)
+            shape = self._in_shape(kind)
+            b = {"bufs": [torch.zeros((1, kmax) + tuple(shape[2:]), dtype=p.storage, device=self.device)
+                          for _ in range(2)],
+                 "out": torch.empty(self._out_shape(kind), dtype=torch.float32, device=self.device),
+                 "host_out": torch.empty(self._out_shape(kind), dtype=torch.float32, pin_memory=True)}
+            torch.cuda.synchronize(self.device)
+            self._slot[key] = b
+            return b
         b = {"in": torch.zeros(self._in_shape(kind), dtype=p.storage, device=self.device),
              "host_out": torch.empty(self._out_shape(kind), dtype=torch.float32, pin_memory=True)}
         if self.use_graphs:
@@ -204,10 +227,18 @@ class GpuExecutor:
         return b
 
     def warm(self, kinds=None):
-        """Allocate buffers and capture graphs for every (slot, class) up front."""
+        """Allocate buffers and capture graphs for every (slot, class) up front; chunked
+        classes run one filter + view-range backprojection on their (zero) buffers."""
+        torch = self.torch
         for s in range(len(self.streams)):
             for k in (kinds or self.plans):
-                self._buffers(s, k)
+                b = self._buffers(s, k)
+                if k in self.chunked:
+                    p = self.plans[k]
+                    lo, hi = p.chunk_bounds(self.chunked[k])[0]
+                    with torch.cuda.stream(self.streams[s]):
+                        p.backproject_views(p.filter(b["bufs"][0][:, :hi - lo]), lo, 2, b["out"])
+        torch.cuda.synchronize(self.device)
 
     def start(self, slot, req):
         torch = self.torch
@@ -215,6 +246,12 @@ class GpuExecutor:
         p = self.plans[req.kind]
         st = self.streams[slot]
         with torch.cuda.stream(st):
+            if req.kind in self.chunked:
+                p.run_chunked(req.payload, b["out"], b["bufs"], self.copy_streams[slot], self.chunked[req.kind])
+                b["host_out"].copy_(b["out"], non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(st)
+                return (ev, b)
             if self.use_graphs:
                 b["static_in"].copy_(req.payload, non_blocking=True)
                 b["graph"].replay()

@@ -69,6 +69,19 @@ def test_invalid_arguments_are_rejected_before_launch():
     assert rc == 1 and "source" in nat.last_error()
 
 
+def test_view_range_backprojection_validation():
+    L = nat.lib()
+    args = lambda mode, ws: (_fake(), 0, 1, 4, 8, 8, _fake(), _fake(), _fake(), _fake(), 4, 4, 4, 0, mode, _fake(),
+                             _fake(), ws, None)
+    assert L.pyronn_cone_backprojection3d_views(*args(4, 1 << 20)) == 1 and "mode" in nat.last_error()
+    assert L.pyronn_cone_backprojection3d_views(*args(-1, 1 << 20)) == 1
+    need = L.pyronn_cone_backprojection3d_workspace_size(1, 4, 8, 8, 0)
+    assert L.pyronn_cone_backprojection3d_views(*args(3, need - 1)) == 2
+    rc = L.pyronn_cone_backprojection3d_views(None, 0, 1, 4, 8, 8, _fake(), _fake(), _fake(), _fake(), 4, 4, 4, 0, 0,
+                                              _fake(), _fake(), need, None)
+    assert rc == 1 and "null" in nat.last_error()
+
+
 def test_too_large_volume_rejected():
     L = nat.lib()
     rc = L.pyronn_cone_projection3d(_fake(), 0, 1, 2048, 2048, 2048, _fake(), _fake(), _fake(), _fake(), _fake(), 2,

@@ -66,3 +66,45 @@ def test_mixed_requests_match_alone(use_graphs, storage):
         if key not in alone:
             alone[key] = plans[q.kind](q.payload.to("cuda")).cpu()
         assert torch.equal(r.output, alone[key]), f"request {q.rid} ({q.kind}) differs from its solo run"
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_chunked_cone_requests_match_alone(storage):
+    """Cone classes served with overlapped view-range uploads (GpuExecutor chunked=...):
+    every result equals the same request reconstructed alone by ConeFDK.run_chunked with
+    the same ranges, bit for bit, and the one-shot plan to the filter GEMM's rounding
+    (rocBLAS may order a GEMM's sums differently for a different number of rows)."""
+    from pyronn_layers.serving import GpuExecutor, Scheduler, latency_report, poisson_arrivals
+    st = torch.float32 if storage == "f32" else torch.bfloat16
+    plans, payloads = _mix(st)
+    chunks = {"cone_s": 2, "cone_m": 3}
+    reqs = poisson_arrivals(4, 400.0, 24, {"fan": 0.3, "cone_s": 0.3, "cone_m": 0.4})
+    for r in reqs:
+        r.payload = payloads[r.kind][r.rid % 2]
+    ex = GpuExecutor(plans, slots=3, use_graphs=True, keep_outputs=True, chunked=chunks)
+    ex.warm()
+    res = Scheduler(ex, slots=3, hbm_budget=1 << 34).run(reqs, idle=lambda dt: None)
+    rep = latency_report(res)
+    print(f"serving chunked {storage}: {rep['completed']} req, p50 {rep['p50_ms']:.2f} ms, p99 {rep['p99_ms']:.2f} ms")
+    assert rep["completed"] == len(reqs)
+    alone = {}
+    for r, q in zip(res, reqs):
+        key = (q.kind, q.rid % 2)
+        if key not in alone:
+            p = plans[q.kind]
+            if q.kind in chunks:
+                g = p.g
+                kmax = max(b - a for a, b in p.chunk_bounds(chunks[q.kind]))
+                bufs = [torch.empty((1, kmax) + tuple(g.detector_shape), dtype=st, device="cuda") for _ in range(2)]
+                out = torch.empty((1,) + tuple(g.volume_shape), device="cuda")
+                p.run_chunked(q.payload, out, bufs, torch.cuda.Stream(), chunks[q.kind])
+                torch.cuda.synchronize()
+                whole = p(q.payload.to("cuda"))
+                # f32: the filter GEMM's rounding only; 16-bit storage: the filtered views are
+                # stored in bf16, where a one-ulp f32 difference can move a value by 2^-8
+                tol = 1e-6 if st == torch.float32 else 1e-4
+                assert ((whole - out).norm() / whole.norm()).item() < tol
+                alone[key] = out.cpu()
+            else:
+                alone[key] = p(q.payload.to("cuda")).cpu()
+        assert torch.equal(r.output, alone[key]), f"request {q.rid} ({q.kind}) differs from its solo run"
