@@ -432,13 +432,21 @@ __device__ __forceinline__ long long claim(unsigned long long *counters, int xcd
 template <typename TS, bool HW>
 __global__ void __launch_bounds__(256)
 cone_fwd_persistent_kernel(FwdBatch B, unsigned long long *counters) {
+  // one claim takes kFwdClaim consecutive items of the XCD's sequence (a 16 x 16 tile of
+  // one view is only tens of microseconds of work, so this amortises the atomic and the
+  // two barriers of a claim); the per-XCD order of items is unchanged
+  constexpr int kFwdClaim = 4;
   const int xcd = (int)(blockIdx.x % kXcds);
   for (;;) {
-    const long long w = claim(counters, xcd) * kXcds + xcd;
-    if (w >= B.total) break;
+    const long long t0 = claim(counters, xcd) * kFwdClaim;
+    if (t0 * kXcds + xcd >= B.total) break;
     int k = 0;
-    while (k + 1 < B.count && w >= B.p[k + 1].begin) ++k;
-    fwd_block<TS, HW>(B.p[k], w - B.p[k].begin);
+    for (int j = 0; j < kFwdClaim; ++j) {
+      const long long w = (t0 + j) * kXcds + xcd;
+      if (w >= B.total) break;
+      while (k + 1 < B.count && w >= B.p[k + 1].begin) ++k;
+      fwd_block<TS, HW>(B.p[k], w - B.p[k].begin);
+    }
   }
 }
 
