@@ -55,6 +55,10 @@ at::Tensor geom(const at::Tensor &t, int64_t B, std::vector<int64_t> shape, cons
   return g.reshape(full);
 }
 
+// An empty output (no batch items, no views, a zero-sized volume or detector) needs no
+// launch; the reference op also returns an empty tensor for an empty batch.
+bool empty_output(const at::Tensor &out) { return out.numel() == 0; }
+
 void check_rc(int rc, const char *op) {
   TORCH_CHECK(rc == PYRONN_OK, op, " failed (status ", rc, "): ", pyronn_last_error());
 }
@@ -89,6 +93,7 @@ at::Tensor cone_projection3d(const at::Tensor &volume, at::IntArrayRef projectio
   const at::Tensor mu = geom(projection_multiplier, B, {1}, op, "projection_multiplier", dev);
   const int dt = dtype_code(vol, op);
   at::Tensor out = at::empty({B, n, Hd, Wd}, vol.options().dtype(at::kFloat));
+  if (empty_output(out)) return out;
   const size_t ws = pyronn_cone_projection3d_workspace_size((int)B, (int)D, (int)H, (int)W, (int)n, dt);
   TORCH_CHECK(ws > 0, op, ": invalid shape");
   at::Tensor work = at::empty({(int64_t)ws}, vol.options().dtype(at::kByte));
@@ -120,6 +125,7 @@ at::Tensor cone_backprojection3d(const at::Tensor &sinogram, at::IntArrayRef vol
   const at::Tensor mu = geom(projection_multiplier, B, {1}, op, "projection_multiplier", dev);
   const int dt = dtype_code(sino, op);
   at::Tensor out = at::empty({B, D, H, W}, sino.options().dtype(at::kFloat));
+  if (empty_output(out)) return out;
   const size_t ws = pyronn_cone_backprojection3d_workspace_size((int)B, (int)n, (int)Hd, (int)Wd, dt);
   TORCH_CHECK(ws > 0, op, ": invalid shape");
   at::Tensor work = at::empty({(int64_t)ws}, sino.options().dtype(at::kByte));
@@ -167,6 +173,7 @@ at::Tensor proj2d(bool fan, const at::Tensor &volume, at::IntArrayRef projection
   Geo2D g = geo2d(B, n, vo, dor, vs, ds, sid, sdd, rays, op, vol.device());
   const int dt = dtype_code(vol, op);
   at::Tensor out = at::empty({B, n, det}, vol.options().dtype(at::kFloat));
+  if (empty_output(out)) return out;
   int rc;
   if (fan)
     rc = pyronn_fan_projection2d(vol.data_ptr(), dt, (int)B, (int)H, (int)W, g.vo.data_ptr<float>(),
@@ -198,6 +205,7 @@ at::Tensor back2d(bool fan, const at::Tensor &sinogram, at::IntArrayRef volume_s
   Geo2D g = geo2d(B, n, vo, dor, vs, ds, sid, sdd, rays, op, sino.device());
   const int dt = dtype_code(sino, op);
   at::Tensor out = at::empty({B, H, W}, sino.options().dtype(at::kFloat));
+  if (empty_output(out)) return out;
   int rc;
   if (fan)
     rc = pyronn_fan_backprojection2d(sino.data_ptr(), dt, (int)B, (int)n, (int)det,

@@ -142,3 +142,25 @@ def test_gradcheck_style_linearity_of_pair():
     x.grad = None
     (pl.cone_projection3d(x, [n, *det], og, sp, P, st, mu) * (2 * y)).sum().backward()
     assert torch.equal(x.grad, 2 * g1)
+
+
+def test_empty_batches_give_empty_outputs():
+    """An empty batch returns an empty output of the right shape (as the reference op does)
+    for all six ops, and an empty batch still validates its geometry."""
+    import pyronn_layers as pl
+    og, sp, P, st, mu = _cone_args(1, 6, (8, 10), (6, 6, 6))
+    vol = torch.zeros((0, 6, 6, 6), device="cuda")
+    assert pl.cone_projection3d(vol, [6, 8, 10], og[0], sp[0], P[0], st[0], mu[0]).shape == (0, 6, 8, 10)
+    sino = torch.zeros((0, 6, 8, 10), device="cuda")
+    assert pl.cone_backprojection3d(sino, [6, 6, 6], og[0], sp[0], P[0], st[0], mu[0]).shape == (0, 6, 6, 6)
+    with pytest.raises(Exception):
+        pl.cone_projection3d(vol, [6, 8, 10], og[0], sp[0], P[0, :3], st[0], mu[0])
+    og2, do, sp2, ds, rays = _par_args(1, 12, 17)
+    img = torch.zeros((0, 16, 16), device="cuda")
+    assert pl.parallel_projection2d(img, [12, 17], og2[0], do[0], sp2[0], ds[0], rays[0]).shape == (0, 12, 17)
+    s2 = torch.zeros((0, 12, 17), device="cuda")
+    assert pl.parallel_backprojection2d(s2, [16, 16], og2[0], do[0], sp2[0], ds[0], rays[0]).shape == (0, 16, 16)
+    sid = torch.full((1,), 90.0, device="cuda")
+    sdd = torch.full((1,), 150.0, device="cuda")
+    assert pl.fan_projection2d(img, [12, 17], og2[0], do[0], sp2[0], ds[0], sid, sdd, rays[0]).shape == (0, 12, 17)
+    assert pl.fan_backprojection2d(s2, [16, 16], og2[0], do[0], sp2[0], ds[0], sid, sdd, rays[0]).shape == (0, 16, 16)
