@@ -51,11 +51,15 @@ __global__ void cosine_weights_kernel(float *__restrict__ w, int Hd, int Wd, dou
 // out[img][r][c] = in[img][r][c] * w[r][c]  (w == nullptr: plain conversion to f32)
 template <typename TS>
 __global__ void weight_kernel(const TS *__restrict__ in, const float *__restrict__ w,
-                              float *__restrict__ out, long long per_image, long long total) {
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-       t += (long long)gridDim.x * blockDim.x) {
+                              float *__restrict__ out, long long per_image, long long images) {
+  // x: pixel within an image, y: image (strided), so the weight index needs no division
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= per_image) return;
+  const float wi = w ? w[i] : 1.0f;
+  for (long long img = blockIdx.y; img < images; img += gridDim.y) {
+    const long long t = img * per_image + i;
     const float x = to_f32(in[t]);
-    out[t] = w ? x * w[t % per_image] : x;
+    out[t] = w ? x * wi : x;
   }
 }
 
@@ -156,15 +160,16 @@ extern "C" int pyronn_fdk_filter(const void *sinogram, int sinogram_dtype, int i
   float *weighted = (float *)workspace;
   float *prod = (float *)((char *)workspace + need / 2);
   const long long per_image = (long long)det_h * det_w;
+  const dim3 wgrid((unsigned)((per_image + 255) / 256), (unsigned)(images < 512 ? images : 512));
   if (sinogram_dtype == PYRONN_F32)
-    weight_kernel<float><<<grid_for(total), 256, 0, s>>>((const float *)sinogram, weights, weighted,
-                                                        per_image, total);
+    weight_kernel<float><<<wgrid, 256, 0, s>>>((const float *)sinogram, weights, weighted, per_image,
+                                               images);
   else if (sinogram_dtype == PYRONN_BF16)
-    weight_kernel<bf16s><<<grid_for(total), 256, 0, s>>>((const bf16s *)sinogram, weights, weighted,
-                                                        per_image, total);
+    weight_kernel<bf16s><<<wgrid, 256, 0, s>>>((const bf16s *)sinogram, weights, weighted, per_image,
+                                               images);
   else
-    weight_kernel<_Float16><<<grid_for(total), 256, 0, s>>>((const _Float16 *)sinogram, weights,
-                                                           weighted, per_image, total);
+    weight_kernel<_Float16><<<wgrid, 256, 0, s>>>((const _Float16 *)sinogram, weights, weighted,
+                                                  per_image, images);
   PYRONN_CHECK_LAUNCH("weight_kernel");
 
   int herr = 0;
