@@ -14,7 +14,7 @@ DT = {"f32": (nat.F32, torch.float32), "bf16": (nat.BF16, torch.bfloat16), "f16"
 
 
 def dev(a, dtype=torch.float32):
-    return torch.as_tensor(np.ascontiguousarray(a)).to("cuda").to(dtype).contiguous()
+    return torch.from_numpy(np.array(a, copy=True, order="C")).to("cuda").to(dtype).contiguous()
 
 
 def p(t):
