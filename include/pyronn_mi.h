@@ -263,6 +263,29 @@ int pyronn_fdk_filter(const void *sinogram, int sinogram_dtype, int images, int 
                       int filtered_dtype, void *workspace, size_t workspace_bytes,
                       pyronn_stream_t stream);
 
+/* The same filter as pyronn_fdk_filter, evaluated by FFT in one fused kernel (no
+ * counterpart in the reference, whose filtering lived outside the op library).
+ * A row of det_w samples is convolved circularly at length L, the smallest power of
+ * two >= max(16, 2 det_w - 1), so the zero padding makes it the linear convolution
+ * with h(c - j), |c - j| < det_w, that T holds.  det_w is limited to 1..4096.
+ *
+ * pyronn_fdk_ramp_spectrum_size: floats of the filter table for det_w (0 outside
+ * the limit).  pyronn_fdk_ramp_spectrum: builds the table (device, 8-byte aligned):
+ * the ramp spectrum scaled by tau / L, the FFT twiddles and the per-output-sample
+ * column weights (NULL: all 1), the FFT counterpart of pyronn_fdk_ramp_matrix.
+ *
+ * pyronn_fdk_filter_fft: filtered[i][r][:] = ((weights[r][:] .* sinogram[i][r][:])
+ * conv h) .* column_weights, stored as filtered_dtype; no workspace.  Rows 2p and
+ * 2p + 1 of an image are filtered together (as one complex sequence), so a row's
+ * result depends only on its own image, whatever the number of images per call.
+ * Results agree with the GEMM filter to float rounding (about 1e-7 relative). */
+size_t pyronn_fdk_ramp_spectrum_size(int det_w);
+int pyronn_fdk_ramp_spectrum(int det_w, double tau, const float *column_weights, float *spectrum,
+                             pyronn_stream_t stream);
+int pyronn_fdk_filter_fft(const void *sinogram, int sinogram_dtype, int images, int det_h,
+                          int det_w, const float *weights, const float *spectrum, void *filtered,
+                          int filtered_dtype, pyronn_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

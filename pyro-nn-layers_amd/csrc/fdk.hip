@@ -15,6 +15,7 @@
 #include <rocblas/rocblas.h>
 
 #include <mutex>
+#include <type_traits>
 #include <unordered_map>
 
 #include "common.h"
@@ -108,6 +109,289 @@ static rocblas_handle blas_handle(int &err) {
   return h;
 }
 
+
+// ----------------------------------------------------------------- FFT ramp filter
+// The same filter as the Toeplitz GEMM, evaluated as a circular convolution of length
+// L = 2^m >= 2 W - 1 (zero padding makes it the linear one, Kak & Slaney ch. 3):
+// Q = IFFT(H . FFT(x)), H[k] = h(0) + 2 sum_{n=1}^{W-1} h(n) cos(2 pi k n / L) (real:
+// the kernel is even).  One workgroup filters two rows of one image at once as the
+// real and imaginary parts of one complex sequence (H is real, so the two stay
+// separate), entirely in LDS: one read of the input and one write of the output.
+// Rows are paired within an image (rows 2p and 2p + 1; an odd last row is paired with
+// zeros), so a row's result depends on its image alone, not on how many images one
+// call filters.
+//
+// FFT: Stockham autosort, radix 8 passes after one radix-2 or radix-4 pass, L / 8
+// lanes each holding 8 values of every pass (one radix-8, two radix-4 or four radix-2
+// butterflies).  The inverse transform is the forward one on conj(H . X) (H folds in
+// 1 / L and tau), conjugated at the end.  The forward transform's first pass reads the
+// weighted input straight from global memory; its last pass leaves the spectrum in
+// natural order in exactly the registers that the inverse transform's first pass (same
+// radix, no twiddles) reads, so the spectral product needs no LDS round trip; the
+// inverse's last pass writes natural order straight to global memory.
+constexpr int kFftMinLog2 = 4, kFftMaxLog2 = 13;  // 16 .. 8192 points, 2 .. 1024 lanes
+
+static int fft_log2(int det_w) {
+  int m = kFftMinLog2;
+  while ((1LL << m) < 2LL * det_w - 1) ++m;
+  return m;
+}
+
+struct cpx {
+  float x, y;
+};
+__device__ __forceinline__ cpx cadd(cpx a, cpx b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cpx csub(cpx a, cpx b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cpx cmul(cpx a, cpx b) {
+  return {fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x)};
+}
+__device__ __forceinline__ cpx mul_mi(cpx a) { return {a.y, -a.x}; }  // a * (-i)
+
+__device__ __forceinline__ void dft2(cpx &a, cpx &b) {
+  const cpx t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+// in place: v0..v3 -> X0..X3, X_k = sum_n v_n e^{-2 pi i k n / 4}
+__device__ __forceinline__ void dft4(cpx &v0, cpx &v1, cpx &v2, cpx &v3) {
+  const cpx a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), a3 = mul_mi(csub(v1, v3));
+  v0 = cadd(a0, a2);
+  v2 = csub(a0, a2);
+  v1 = cadd(a1, a3);
+  v3 = csub(a1, a3);
+}
+__device__ __forceinline__ void dft8(cpx *v) {
+  cpx e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  cpx o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4(e0, e1, e2, e3);
+  dft4(o0, o1, o2, o3);
+  const float s = 0.70710678118654752f;  // e^{-2 pi i k / 8}, k = 1, 2, 3 applied to o1..o3
+  o1 = {(o1.x + o1.y) * s, (o1.y - o1.x) * s};
+  o2 = mul_mi(o2);
+  o3 = {(o3.y - o3.x) * s, -(o3.x + o3.y) * s};
+  v[0] = cadd(e0, o0);
+  v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1);
+  v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2);
+  v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3);
+  v[7] = csub(e3, o3);
+}
+
+// Radix of pass i of an L = 2^m transform: one radix-2 / radix-4 pass first when m is
+// not a multiple of 3, radix 8 after.
+__device__ __forceinline__ int fft_radix(int m, int i) {
+  const int r = m % 3;
+  if (r == 0 || i > 0) return 8;
+  return r == 1 ? 2 : 4;
+}
+
+// Calls f with the radix as a compile-time constant, so that every index into a lane's
+// eight values is a constant and the values stay in registers.
+template <typename F>
+__device__ __forceinline__ void with_radix(int R, F f) {
+  if (R == 8)
+    f(std::integral_constant<int, 8>{});
+  else if (R == 4)
+    f(std::integral_constant<int, 4>{});
+  else
+    f(std::integral_constant<int, 2>{});
+}
+
+// Butterfly b (< 8 / R) of a lane in a pass of radix R has index j = lane + b L / 8; its
+// values v[b R + r] come from x[j + r L / R] and go to x[(j / Ns) Ns R + (j % Ns) + r Ns],
+// Ns the product of the earlier radices.
+template <int R>
+__device__ __forceinline__ void fft_twiddle_dft(cpx *v, int Ns, int lane, int T, int L,
+                                                const cpx *__restrict__ tw) {
+  if (Ns > 1) {
+#pragma unroll
+    for (int b = 0; b < 8 / R; ++b) {
+      const int k = (lane + b * T) & (Ns - 1);
+      const int step = k * (L / (Ns * R));
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], tw[r * step]);
+    }
+  }
+  if constexpr (R == 8) {
+    dft8(v);
+  } else if constexpr (R == 4) {
+    dft4(v[0], v[1], v[2], v[3]);
+    dft4(v[4], v[5], v[6], v[7]);
+  } else {
+    dft2(v[0], v[1]);
+    dft2(v[2], v[3]);
+    dft2(v[4], v[5]);
+    dft2(v[6], v[7]);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void fft_store(const cpx *v, float *re, float *im, int Ns, int lane,
+                                          int T) {
+#pragma unroll
+  for (int b = 0; b < 8 / R; ++b) {
+    const int j = lane + b * T;
+    const int base = (j & ~(Ns - 1)) * R + (j & (Ns - 1));
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      re[base + r * Ns] = v[b * R + r].x;
+      im[base + r * Ns] = v[b * R + r].y;
+    }
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void fft_load(cpx *v, const float *re, const float *im, int lane, int T,
+                                         int L) {
+#pragma unroll
+  for (int b = 0; b < 8 / R; ++b) {
+    const int j = lane + b * T;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[b * R + r] = {re[j + r * (L / R)], im[j + r * (L / R)]};
+  }
+}
+
+// Natural-order position of value q of a lane before a first pass / after a last pass of
+// radix R: j + r L / R with j = lane + b L / 8, q = b R + r.
+template <int R>
+__device__ __forceinline__ int fft_natural(int q, int lane, int T, int L) {
+  return lane + (q / R) * T + (q % R) * (L / R);
+}
+
+// spec layout (floats): [0, L) H[k] tau / L; [L, 3L) twiddles e^{-2 pi i m / L} as
+// (re, im) pairs; [3L, 3L + W) column weights.
+template <typename TI, typename TO>
+__global__ void __launch_bounds__(1024)
+    ramp_fft_kernel(const TI *__restrict__ in, const float *__restrict__ w,
+                    const float *__restrict__ spec, TO *__restrict__ out, int det_h, int W, int m,
+                    int pairs) {
+  extern __shared__ float lds[];
+  const int L = 1 << m, T = L >> 3, lane = threadIdx.x;
+  float *re = lds, *im = lds + L;
+  const float *H = spec;
+  const cpx *tw = reinterpret_cast<const cpx *>(spec + L);
+  const float *colw = spec + 3 * L;
+  const int img = (int)(blockIdx.x / (unsigned)pairs);
+  const int ra = 2 * (int)(blockIdx.x - (unsigned)img * (unsigned)pairs);
+  const bool has_b = ra + 1 < det_h;
+  const size_t row_a = ((size_t)img * det_h + ra) * (size_t)W;
+  const float *wa = w ? w + (size_t)ra * W : nullptr;
+  const int passes = (m + 2) / 3;
+  cpx v[8];
+
+  // forward transform; its first pass reads the weighted, zero-padded rows from global memory
+  int R = fft_radix(m, 0), Ns = 1;
+  with_radix(R, [&](auto rc) {
+    constexpr int RR = decltype(rc)::value;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int c = fft_natural<RR>(q, lane, T, L);
+      float a = 0.0f, b = 0.0f;
+      if (c < W) {
+        a = to_f32(in[row_a + c]);
+        if (wa) a = a * wa[c];
+        if (has_b) {
+          b = to_f32(in[row_a + W + c]);
+          if (wa) b = b * wa[W + c];
+        }
+      }
+      v[q] = {a, b};
+    }
+    fft_twiddle_dft<RR>(v, Ns, lane, T, L, tw);
+  });
+  for (int i = 1; i < passes; ++i) {
+    if (i > 1) __syncthreads();
+    with_radix(R, [&](auto rc) { fft_store<decltype(rc)::value>(v, re, im, Ns, lane, T); });
+    __syncthreads();
+    Ns *= R;
+    R = fft_radix(m, i);
+    with_radix(R, [&](auto rc) {
+      constexpr int RR = decltype(rc)::value;
+      fft_load<RR>(v, re, im, lane, T, L);
+      fft_twiddle_dft<RR>(v, Ns, lane, T, L, tw);
+    });
+  }
+  // the spectrum is in natural order in v: replace X[f] by conj(H[f] X[f]) (H is real), then
+  // run the inverse's first pass (radix R, Ns = 1) on the same values
+  Ns = 1;
+  with_radix(R, [&](auto rc) {
+    constexpr int RR = decltype(rc)::value;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const float h = H[fft_natural<RR>(q, lane, T, L)];
+      v[q] = {h * v[q].x, -(h * v[q].y)};
+    }
+    fft_twiddle_dft<RR>(v, Ns, lane, T, L, tw);
+  });
+  for (int i = passes - 2; i >= 0; --i) {
+    __syncthreads();
+    with_radix(R, [&](auto rc) { fft_store<decltype(rc)::value>(v, re, im, Ns, lane, T); });
+    __syncthreads();
+    Ns *= R;
+    R = fft_radix(m, i);
+    with_radix(R, [&](auto rc) {
+      constexpr int RR = decltype(rc)::value;
+      fft_load<RR>(v, re, im, lane, T, L);
+      fft_twiddle_dft<RR>(v, Ns, lane, T, L, tw);
+    });
+  }
+  // natural order again; the filtered rows are conj(v)
+  with_radix(R, [&](auto rc) {
+    constexpr int RR = decltype(rc)::value;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int c = fft_natural<RR>(q, lane, T, L);
+      if (c < W) {
+        store_as(v[q].x * colw[c], out + row_a + c);
+        if (has_b) store_as(-v[q].y * colw[c], out + row_a + W + c);
+      }
+    }
+  });
+}
+
+// One lane per table entry; double precision throughout.
+__global__ void ramp_spectrum_kernel(float *__restrict__ spec, int W, int m, double tau,
+                                     const float *__restrict__ colw) {
+  const int L = 1 << m;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const double pi = 3.14159265358979323846;
+  if (t < L) {
+    // H[k] = h(0) + 2 sum_{odd n < W} h(n) cos(2 pi k n / L), h(n) = -1 / (n pi tau)^2
+    double acc = 0.0;
+    for (int n = 1; n < W; n += 2) {
+      const long long kn = ((long long)t * n) & (L - 1);  // the cosine's period is L
+      acc += cospi(2.0 * (double)kn / (double)L) / ((double)n * (double)n);
+    }
+    const double H = 1.0 / (4.0 * tau * tau) - 2.0 * acc / (pi * pi * tau * tau);
+    spec[t] = (float)(H * tau / (double)L);
+    double sn, cs;
+    sincospi(2.0 * (double)t / (double)L, &sn, &cs);
+    spec[L + 2 * t] = (float)cs;
+    spec[L + 2 * t + 1] = (float)(-sn);
+  }
+  if (t < W) spec[3 * L + t] = colw ? colw[t] : 1.0f;
+}
+
+template <typename TI>
+static void launch_ramp_fft(const TI *in, const float *w, const float *spec, void *out, int odt,
+                            int images, int det_h, int W, int m, hipStream_t s) {
+  const int pairs = (det_h + 1) / 2;
+  const unsigned blocks = (unsigned)((long long)images * pairs);
+  const unsigned threads = 1u << (m - 3);
+  const size_t lds = (size_t)8 << m;
+  if (odt == PYRONN_F32)
+    ramp_fft_kernel<TI, float><<<blocks, threads, lds, s>>>(in, w, spec, (float *)out, det_h, W, m,
+                                                            pairs);
+  else if (odt == PYRONN_BF16)
+    ramp_fft_kernel<TI, bf16s><<<blocks, threads, lds, s>>>(in, w, spec, (bf16s *)out, det_h, W, m,
+                                                            pairs);
+  else
+    ramp_fft_kernel<TI, _Float16><<<blocks, threads, lds, s>>>(in, w, spec, (_Float16 *)out, det_h,
+                                                               W, m, pairs);
+}
+
 }  // namespace pyronn
 
 using namespace pyronn;
@@ -195,5 +479,54 @@ extern "C" int pyronn_fdk_filter(const void *sinogram, int sinogram_dtype, int i
   else if (filtered_dtype == PYRONN_F16)
     convert_kernel<_Float16><<<grid_for(total), 256, 0, s>>>(prod, (_Float16 *)filtered, total);
   PYRONN_CHECK_LAUNCH("convert_kernel");
+  return PYRONN_OK;
+}
+
+extern "C" size_t pyronn_fdk_ramp_spectrum_size(int det_w) {
+  if (det_w <= 0 || det_w > (1 << (kFftMaxLog2 - 1))) return 0;
+  return (size_t)3 * ((size_t)1 << fft_log2(det_w)) + (size_t)det_w;
+}
+
+extern "C" int pyronn_fdk_ramp_spectrum(int det_w, double tau, const float *column_weights,
+                                        float *spectrum, pyronn_stream_t stream) {
+  clear_error();
+  PYRONN_CHECK_ARG(spectrum, "fdk_ramp_spectrum: null pointer");
+  PYRONN_CHECK_ARG(pyronn_fdk_ramp_spectrum_size(det_w) > 0,
+                   "fdk_ramp_spectrum: det_w %d outside 1..%d", det_w, 1 << (kFftMaxLog2 - 1));
+  PYRONN_CHECK_ARG(tau > 0.0, "fdk_ramp_spectrum: sample spacing must be > 0");
+  PYRONN_CHECK_ARG(((uintptr_t)spectrum & 7) == 0, "fdk_ramp_spectrum: table must be 8-byte aligned");
+  const int m = fft_log2(det_w);
+  const int n = (1 << m) > det_w ? (1 << m) : det_w;
+  ramp_spectrum_kernel<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(spectrum, det_w, m, tau,
+                                                                         column_weights);
+  PYRONN_CHECK_LAUNCH("ramp_spectrum_kernel");
+  return PYRONN_OK;
+}
+
+extern "C" int pyronn_fdk_filter_fft(const void *sinogram, int sinogram_dtype, int images,
+                                     int det_h, int det_w, const float *weights,
+                                     const float *spectrum, void *filtered, int filtered_dtype,
+                                     pyronn_stream_t stream) {
+  clear_error();
+  PYRONN_CHECK_ARG(sinogram && spectrum && filtered, "fdk_filter_fft: null pointer");
+  PYRONN_CHECK_ARG(dtype_ok(sinogram_dtype) && dtype_ok(filtered_dtype), "fdk_filter_fft: bad dtype");
+  PYRONN_CHECK_ARG(images > 0 && det_h > 0 && det_w > 0, "fdk_filter_fft: bad shape");
+  PYRONN_CHECK_ARG(pyronn_fdk_ramp_spectrum_size(det_w) > 0,
+                   "fdk_filter_fft: det_w %d outside 1..%d", det_w, 1 << (kFftMaxLog2 - 1));
+  PYRONN_CHECK_ARG((long long)images * ((det_h + 1) / 2) <= INT32_MAX,
+                   "fdk_filter_fft: too many row pairs for one launch");
+  PYRONN_CHECK_ARG(((uintptr_t)spectrum & 7) == 0, "fdk_filter_fft: table must be 8-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  const int m = fft_log2(det_w);
+  if (sinogram_dtype == PYRONN_F32)
+    launch_ramp_fft((const float *)sinogram, weights, spectrum, filtered, filtered_dtype, images,
+                    det_h, det_w, m, s);
+  else if (sinogram_dtype == PYRONN_BF16)
+    launch_ramp_fft((const bf16s *)sinogram, weights, spectrum, filtered, filtered_dtype, images,
+                    det_h, det_w, m, s);
+  else
+    launch_ramp_fft((const _Float16 *)sinogram, weights, spectrum, filtered, filtered_dtype, images,
+                    det_h, det_w, m, s);
+  PYRONN_CHECK_LAUNCH("ramp_fft_kernel");
   return PYRONN_OK;
 }

@@ -20,10 +20,11 @@ formulated after Kak & Slaney, ch. 3, for full (2 pi) orbits:
         Q = R filtered at tau = detector spacing.
 
 Plans keep all geometry and filter state resident on the device (projection
-matrices, cosine weights, the Toeplitz ramp matrix, workspaces), so one
-reconstruction is: one weighting kernel + one rocBLAS GEMM (+ a storage
-conversion for BF16/FP16) + one backprojection, all on the current stream and
-capturable in a HIP graph (see `capture`).
+matrices, cosine weights, the ramp spectrum and FFT twiddles), so one
+reconstruction is: one fused weighting + FFT ramp filter + storage conversion
+kernel and one backprojection, all on the current stream and capturable in a HIP
+graph (see `capture`).  Rows wider than 4096 samples take the Toeplitz-matrix
+GEMM filter (weighting kernel + rocBLAS SGEMM + conversion) instead.
 """
 from __future__ import annotations
 
@@ -51,6 +52,33 @@ def _stream():
 class _Plan:
     storage: torch.dtype
     device: torch.device
+    # "fft": the fused LDS FFT filter (detector rows up to 4096 samples); "gemm": the
+    # Toeplitz-matrix product in rocBLAS (any width).  Both compute the same convolution.
+    filter_impl: str = "fft"
+
+    def _init_filter(self, det_w: int, tau: float, column_weights: torch.Tensor | None = None):
+        self.tau = tau
+        self._det_w = det_w
+        self._colw = column_weights
+        self.ramp = None
+        self.spectrum = None
+        n = nat.lib().pyronn_fdk_ramp_spectrum_size(det_w)
+        if n > 0:
+            self.spectrum = torch.empty(n, dtype=torch.float32, device=self.device)
+            nat.call("pyronn_fdk_ramp_spectrum", det_w, tau, _p(column_weights) if column_weights is not None
+                     else None, _p(self.spectrum), _stream())
+        else:
+            self.filter_impl = "gemm"
+            self.ramp_matrix()
+
+    def ramp_matrix(self) -> torch.Tensor:
+        """The Toeplitz ramp matrix of the GEMM filter (built on first use)."""
+        if self.ramp is None:
+            W = self._det_w
+            self.ramp = torch.empty((W, W), dtype=torch.float32, device=self.device)
+            nat.call("pyronn_fdk_ramp_matrix", W, self.tau, _p(self._colw) if self._colw is not None else None,
+                     _p(self.ramp), _stream())
+        return self.ramp
 
     def _filter(self, sino: torch.Tensor, images: int, det_h: int, det_w: int) -> torch.Tensor:
         sino = sino.contiguous()
@@ -58,12 +86,17 @@ class _Plan:
         if code is None:
             raise TypeError(f"sinogram dtype {sino.dtype} not supported")
         out = torch.empty(sino.shape, dtype=self.storage, device=self.device)
+        w = _p(self.weights) if self.weights is not None else None
+        if self.filter_impl == "fft":
+            nat.call("pyronn_fdk_filter_fft", _p(sino), code, images, det_h, det_w, w, _p(self.spectrum), _p(out),
+                     _DT[self.storage], _stream())
+            return out
+        ramp = self.ramp_matrix()
         # Per-call workspace from torch's stream-aware caching allocator (graph-pool
         # memory under capture), so concurrent calls on different streams never share it.
         ws_bytes = nat.lib().pyronn_fdk_filter_workspace_size(images, det_h, det_w)
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=self.device)
-        w = _p(self.weights) if self.weights is not None else None
-        nat.call("pyronn_fdk_filter", _p(sino), code, images, det_h, det_w, w, _p(self.ramp), _p(out),
+        nat.call("pyronn_fdk_filter", _p(sino), code, images, det_h, det_w, w, _p(ramp), _p(out),
                  _DT[self.storage], _p(ws), ws.numel(), _stream())
         return out
 
@@ -107,9 +140,7 @@ class ConeFDK(_Plan):
         self.weights = torch.empty((Hd, Wd), dtype=torch.float32, device=dev)
         nat.call("pyronn_fdk_cosine_weights", Hd, Wd, du, dv, (Wd - 1) / 2.0, (Hd - 1) / 2.0, g.sdd,
                  _p(self.weights), _stream())
-        self.tau = du * g.sid / g.sdd
-        self.ramp = torch.empty((Wd, Wd), dtype=torch.float32, device=dev)
-        nat.call("pyronn_fdk_ramp_matrix", Wd, self.tau, None, _p(self.ramp), _stream())
+        self._init_filter(Wd, du * g.sid / g.sdd)
 
     def filter(self, sino: torch.Tensor) -> torch.Tensor:
         B, n, Hd, Wd = sino.shape
@@ -196,9 +227,7 @@ class FanFBP(_Plan):
         # full orbit: (1/2) dbeta = pi / n, which the reference already applies.
         post = ((g.sdd ** 2 + s ** 2) / g.sdd ** 2) * (g.sid / g.sdd) * (g.angular_range / (2 * math.pi))
         colw = torch.tensor(post, dtype=torch.float32, device=dev)
-        self.tau = ds * g.sid / g.sdd
-        self.ramp = torch.empty((det, det), dtype=torch.float32, device=dev)
-        nat.call("pyronn_fdk_ramp_matrix", det, self.tau, _p(colw), _p(self.ramp), _stream())
+        self._init_filter(det, ds * g.sid / g.sdd, colw)
         torch.cuda.current_stream().synchronize()
 
     def __call__(self, sino: torch.Tensor) -> torch.Tensor:
@@ -228,11 +257,9 @@ class ParallelFBP(_Plan):
         self.do = torch.full((1, 1), g.detector_origin, dtype=torch.float32, device=dev)
         self.ds = torch.full((1, 1), ds, dtype=torch.float32, device=dev)
         self.weights = None
-        self.tau = ds
         # The reference's pi / n scaling is the FBP weight for views over pi and,
         # since every direction then appears twice, also for views over 2 pi.
-        self.ramp = torch.empty((det, det), dtype=torch.float32, device=dev)
-        nat.call("pyronn_fdk_ramp_matrix", det, self.tau, None, _p(self.ramp), _stream())
+        self._init_filter(det, ds)
         torch.cuda.current_stream().synchronize()
 
     def __call__(self, sino: torch.Tensor) -> torch.Tensor:

@@ -60,6 +60,9 @@ _SIGS = {
     "pyronn_fdk_cosine_weights": (_i, [_i, _i] + [ctypes.c_double] * 5 + [_vp, _vp]),
     "pyronn_fdk_filter_workspace_size": (_sz, [_i, _i, _i]),
     "pyronn_fdk_filter": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _sz, _vp]),
+    "pyronn_fdk_ramp_spectrum_size": (_sz, [_i]),
+    "pyronn_fdk_ramp_spectrum": (_i, [_i, ctypes.c_double, _vp, _vp, _vp]),
+    "pyronn_fdk_filter_fft": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp]),
 }
 
 EXPORTED = tuple(_SIGS)

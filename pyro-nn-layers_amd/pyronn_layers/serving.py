@@ -6,7 +6,7 @@ admitted as soon as (a) it has arrived, (b) an execution slot is free and
 already running.  Each slot owns a HIP stream (default priority) and its own
 device buffers, so admitted requests run concurrently on the device: the
 upload of one overlaps the backprojection of another.  Plans (projection
-matrices, cosine weights, ramp matrices) are built once per geometry class and
+matrices, cosine weights, ramp filter tables) are built once per geometry class and
 stay resident; with `use_graphs` each slot replays a HIP graph captured per
 class instead of launching the pipeline kernel by kernel.
 
@@ -189,12 +189,13 @@ class GpuExecutor:
         nin = int(np.prod(self._in_shape(kind)))
         nout = int(np.prod(self._out_shape(kind)))
         e = p.storage.itemsize
+        ws = 8 if getattr(p, "filter_impl", "gemm") == "gemm" else 0   # GEMM filter workspace: 2 x f32
         if kind in self.chunked:
             # two range buffers + filtered range + filter workspace + detector cells of one range + output
             nk = -(-nin // self.chunked[kind])
-            return nk * (2 * e + e + 8 + 4 * e) + nout * 4
-        # input + filtered + filter workspace (2 x f32) + detector cells (4 x) + output
-        return nin * (e + e + 8 + 4 * e) + nout * 4
+            return nk * (2 * e + e + ws + 4 * e) + nout * 4
+        # input + filtered + filter workspace + detector cells (4 x) + output
+        return nin * (e + e + ws + 4 * e) + nout * 4
 
     def footprint(self, kind) -> int:
         return self._fp[kind]

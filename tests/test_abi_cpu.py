@@ -138,3 +138,26 @@ def test_batched_argument_validation():
     need = L.pyronn_cone_projection3d_batched_workspace_size(arr, 2, 0)
     assert need == 256 + 2 * L.pyronn_cone_projection3d_workspace_size(1, 4, 4, 4, 2, 0)
     assert L.pyronn_cone_projection3d_batched(arr, 2, 0, 0, 0, _fake(), need - 1, None) == 2
+
+
+def test_fft_filter_limits_and_validation():
+    L = nat.lib()
+    # table: ramp spectrum (L) + twiddles (2 L) + column weights (W), L = pow2 >= max(16, 2 W - 1)
+    assert L.pyronn_fdk_ramp_spectrum_size(1) == 3 * 16 + 1
+    assert L.pyronn_fdk_ramp_spectrum_size(9) == 3 * 32 + 9
+    assert L.pyronn_fdk_ramp_spectrum_size(1024) == 3 * 2048 + 1024
+    assert L.pyronn_fdk_ramp_spectrum_size(4096) == 3 * 8192 + 4096
+    assert L.pyronn_fdk_ramp_spectrum_size(4097) == 0
+    assert L.pyronn_fdk_ramp_spectrum_size(0) == 0
+    assert L.pyronn_fdk_ramp_spectrum(4097, 1.0, None, _fake(), None) == 1 and "outside" in nat.last_error()
+    assert L.pyronn_fdk_ramp_spectrum(64, 0.0, None, _fake(), None) == 1 and "spacing" in nat.last_error()
+    assert L.pyronn_fdk_ramp_spectrum(64, 1.0, None, ctypes.c_void_p(0x1004), None) == 1
+    assert "aligned" in nat.last_error()
+    args = lambda imgs, h, w, dt: (_fake(), dt, imgs, h, w, None, _fake(2), _fake(3), 0, None)
+    assert L.pyronn_fdk_filter_fft(*args(1, 4, 5000, 0)) == 1 and "outside" in nat.last_error()
+    assert L.pyronn_fdk_filter_fft(*args(0, 4, 64, 0)) == 1 and "shape" in nat.last_error()
+    assert L.pyronn_fdk_filter_fft(*args(1, 4, 64, 9)) == 1 and "dtype" in nat.last_error()
+    assert L.pyronn_fdk_filter_fft(*args(1 << 30, 8, 64, 0)) == 1 and "pairs" in nat.last_error()
+    assert L.pyronn_fdk_filter_fft(None, 0, 1, 4, 64, None, _fake(2), _fake(3), 0, None) == 1
+    assert "null" in nat.last_error()
+

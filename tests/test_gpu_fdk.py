@@ -49,6 +49,79 @@ def test_cone_filter_matches_numpy_convolution(cone_case):
     assert err < 2e-6, err
 
 
+def _ramp_rows(x, tau, weights=None, colw=None):
+    x = x.astype(np.float64) * (1.0 if weights is None else weights.astype(np.float64))
+    y = filter_rows(x, tau)
+    return y if colw is None else y * colw
+
+
+@pytest.mark.parametrize("det_w", [1, 7, 16, 33, 400, 725, 1024, 4096])
+@pytest.mark.parametrize("det_h", [1, 4, 5])
+def test_fft_filter_matches_convolution_and_gemm(det_w, det_h):
+    """The fused FFT filter (weighting, zero-padded circular convolution, column weights)
+    against the float64 linear convolution and against the Toeplitz GEMM filter, for
+    widths at and between the FFT sizes' limits, odd row counts (a row paired with
+    zeros) and single rows."""
+    from pyronn_layers import native as nat
+    from pyronn_layers.fdk import _p, _stream
+    rng = np.random.default_rng(det_w * 10 + det_h)
+    images = 3 if det_w <= 1024 else 2
+    x = rng.normal(size=(images, det_h, det_w)).astype(np.float32)
+    wts = rng.uniform(0.5, 1.0, (det_h, det_w)).astype(np.float32)
+    colw = rng.uniform(0.5, 1.5, det_w).astype(np.float32)
+    tau = 0.85
+    dev = torch.device("cuda")
+    tx, tw, tc = (torch.tensor(a, device=dev) for a in (x, wts, colw))
+    spec = torch.empty(nat.lib().pyronn_fdk_ramp_spectrum_size(det_w), device=dev)
+    nat.call("pyronn_fdk_ramp_spectrum", det_w, tau, _p(tc), _p(spec), _stream())
+    out = torch.empty_like(tx)
+    nat.call("pyronn_fdk_filter_fft", _p(tx), nat.F32, images, det_h, det_w, _p(tw), _p(spec), _p(out), nat.F32,
+             _stream())
+    T = torch.empty((det_w, det_w), device=dev)
+    nat.call("pyronn_fdk_ramp_matrix", det_w, tau, _p(tc), _p(T), _stream())
+    ws_n = nat.lib().pyronn_fdk_filter_workspace_size(images, det_h, det_w)
+    ws = torch.empty(ws_n, dtype=torch.uint8, device=dev)
+    gemm = torch.empty_like(tx)
+    nat.call("pyronn_fdk_filter", _p(tx), nat.F32, images, det_h, det_w, _p(tw), _p(T), _p(gemm), nat.F32, _p(ws),
+             ws_n, _stream())
+    torch.cuda.synchronize()
+    ref = _ramp_rows(x, tau, wts[None], colw)
+    got, gg = out.cpu().numpy(), gemm.cpu().numpy()
+    rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    rel_gemm = np.linalg.norm(gg - ref) / np.linalg.norm(ref)
+    print(f"W={det_w} H={det_h}: FFT rel L2 {rel:.2e}, GEMM rel L2 {rel_gemm:.2e}")
+    assert rel < 1e-6, rel
+    assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
+    # one image at a time gives the same bits: a row's result depends on its image only
+    for i in range(images):
+        one = torch.empty_like(tx[i:i + 1])
+        nat.call("pyronn_fdk_filter_fft", _p(tx[i:i + 1].contiguous()), nat.F32, 1, det_h, det_w, _p(tw), _p(spec),
+                 _p(one), nat.F32, _stream())
+        torch.cuda.synchronize()
+        assert torch.equal(one, out[i:i + 1])
+
+
+@pytest.mark.parametrize("src,dst", [(torch.bfloat16, torch.float32), (torch.float32, torch.bfloat16),
+                                     (torch.float16, torch.float16), (torch.bfloat16, torch.bfloat16)])
+def test_fft_filter_storage_types(src, dst):
+    """16-bit inputs are read exactly (widened to f32); 16-bit outputs are the f32 result
+    rounded to nearest even."""
+    from pyronn_layers import native as nat
+    from pyronn_layers.fdk import _DT, _p, _stream
+    rng = np.random.default_rng(3)
+    det_h, det_w = 6, 300
+    x = torch.tensor(rng.normal(size=(2, det_h, det_w)).astype(np.float32), device="cuda").to(src)
+    spec = torch.empty(nat.lib().pyronn_fdk_ramp_spectrum_size(det_w), device="cuda")
+    nat.call("pyronn_fdk_ramp_spectrum", det_w, 1.0, None, _p(spec), _stream())
+    f32 = torch.empty(x.shape, device="cuda")
+    nat.call("pyronn_fdk_filter_fft", _p(x.float().contiguous()), nat.F32, 2, det_h, det_w, None, _p(spec), _p(f32),
+             nat.F32, _stream())
+    out = torch.empty(x.shape, dtype=dst, device="cuda")
+    nat.call("pyronn_fdk_filter_fft", _p(x), _DT[src], 2, det_h, det_w, None, _p(spec), _p(out), _DT[dst], _stream())
+    torch.cuda.synchronize()
+    assert torch.equal(out, f32.to(dst))
+
+
 def test_cone_fdk_matches_oracle_pipeline_and_recovers_density(cone_case):
     from pyronn_layers.fdk import ConeFDK
     g, vol, sino = cone_case
@@ -168,7 +241,8 @@ def test_view_range_backprojection_chains_to_the_one_pass_result(cone_case, stor
 def test_chunked_fdk_from_pinned_host_memory(cone_case):
     """ConeFDK.run_chunked uploads view ranges on a second stream while the first filters
     and backprojects: its result equals filtering each range and backprojecting all views
-    in one pass bit for bit, and the one-shot plan up to the filter GEMM's rounding."""
+    in one pass bit for bit, and (the FFT filter treats every image alone) the one-shot
+    plan too; the GEMM filter's rounding may depend on the row count of a call."""
     from pyronn_layers.fdk import ConeFDK
     g, _, sino = cone_case
     dev = torch.device("cuda")
@@ -187,6 +261,10 @@ def test_chunked_fdk_from_pinned_host_memory(cone_case):
                                                         False)
     assert torch.equal(out, ref)
     whole = plan(dsino)
+    assert plan.filter_impl == "fft"
+    assert torch.equal(whole, out)
+    plan.filter_impl = "gemm"
+    whole = plan(dsino)
     rel = ((whole - out).norm() / whole.norm()).item()
-    print(f"chunked vs one-shot FDK: rel L2 {rel:.2e}, bit-identical: {torch.equal(whole, out)}")
+    print(f"chunked FFT-filtered vs one-shot GEMM-filtered FDK: rel L2 {rel:.2e}")
     assert rel < 1e-6
