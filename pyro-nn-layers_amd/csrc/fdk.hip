@@ -208,10 +208,25 @@ __device__ __forceinline__ void fft_twiddle_dft(cpx *v, int Ns, int lane, int T,
   if (Ns > 1) {
 #pragma unroll
     for (int b = 0; b < 8 / R; ++b) {
+      // w^r, w = e^{-2 pi i k / (Ns R)}: w and w^2 from the table, higher powers as
+      // products of at most two table values or one earlier product (two table loads
+      // per butterfly instead of R - 1)
       const int k = (lane + b * T) & (Ns - 1);
       const int step = k * (L / (Ns * R));
-#pragma unroll
-      for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], tw[r * step]);
+      const cpx w1 = tw[step];
+      v[b * R + 1] = cmul(v[b * R + 1], w1);
+      if constexpr (R >= 4) {
+        const cpx w2 = tw[2 * step], w3 = cmul(w2, w1);
+        v[b * R + 2] = cmul(v[b * R + 2], w2);
+        v[b * R + 3] = cmul(v[b * R + 3], w3);
+        if constexpr (R == 8) {
+          const cpx w4 = cmul(w2, w2);
+          v[b * R + 4] = cmul(v[b * R + 4], w4);
+          v[b * R + 5] = cmul(v[b * R + 5], cmul(w4, w1));
+          v[b * R + 6] = cmul(v[b * R + 6], cmul(w4, w2));
+          v[b * R + 7] = cmul(v[b * R + 7], cmul(w4, w3));
+        }
+      }
     }
   }
   if constexpr (R == 8) {
@@ -227,6 +242,14 @@ __device__ __forceinline__ void fft_twiddle_dft(cpx *v, int Ns, int lane, int T,
   }
 }
 
+// LDS word of sequence position p: the low five bits are XORed with a function of the
+// 32-word row, a bijection within each row that spreads the strided stores of the
+// early passes over all banks (contiguous accesses stay conflict free).
+__device__ __forceinline__ int lds_at(int p) {
+  const int row = p >> 5;
+  return p ^ ((row ^ (row << 2)) & 31);
+}
+
 template <int R>
 __device__ __forceinline__ void fft_store(const cpx *v, float *re, float *im, int Ns, int lane,
                                           int T) {
@@ -236,8 +259,9 @@ __device__ __forceinline__ void fft_store(const cpx *v, float *re, float *im, in
     const int base = (j & ~(Ns - 1)) * R + (j & (Ns - 1));
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      re[base + r * Ns] = v[b * R + r].x;
-      im[base + r * Ns] = v[b * R + r].y;
+      const int a = lds_at(base + r * Ns);
+      re[a] = v[b * R + r].x;
+      im[a] = v[b * R + r].y;
     }
   }
 }
@@ -249,7 +273,10 @@ __device__ __forceinline__ void fft_load(cpx *v, const float *re, const float *i
   for (int b = 0; b < 8 / R; ++b) {
     const int j = lane + b * T;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[b * R + r] = {re[j + r * (L / R)], im[j + r * (L / R)]};
+    for (int r = 0; r < R; ++r) {
+      const int a = lds_at(j + r * (L / R));
+      v[b * R + r] = {re[a], im[a]};
+    }
   }
 }
 
