@@ -22,8 +22,9 @@
  *    PYRONN_BF16 or PYRONN_F16.  Arithmetic is float32 in every case.
  *  - Calls are asynchronous on `stream` (NULL = the null stream), allocate no
  *    device memory and keep no state between calls: the caller passes the
- *    workspace sized by the matching *_workspace_size() function.  Calls are
- *    reentrant; two calls may run concurrently if their workspaces differ.
+ *    workspace sized by the matching *_workspace_size() function, 16-byte
+ *    aligned (every device allocation is).  Calls are reentrant; two calls may
+ *    run concurrently if their workspaces differ.
  *  - Return value: PYRONN_OK or an error code; pyronn_last_error() gives the
  *    message for the calling thread.  Invalid arguments are rejected before
  *    anything is launched.  Invalid *values* that only the device sees (a

@@ -463,26 +463,50 @@ __device__ __forceinline__ TS pix(const TS *__restrict__ s, int u, int v, int Wd
   return s[(size_t)v * Wd + u];
 }
 
+// The four neighbours of a cell in one store: 16 bytes (f32) or 8 bytes (16-bit types).
+__device__ __forceinline__ void store_cell(float *o, float a, float b, float c, float d) {
+  *reinterpret_cast<float4 *>(o) = make_float4(a, b, c, d);
+}
+template <typename TS>
+__device__ __forceinline__ void store_cell(TS *o, TS a, TS b, TS c, TS d) {
+  static_assert(sizeof(TS) == 2, "16-bit storage");
+  uint16_t h[4];
+  __builtin_memcpy(&h[0], &a, 2);
+  __builtin_memcpy(&h[1], &b, 2);
+  __builtin_memcpy(&h[2], &c, 2);
+  __builtin_memcpy(&h[3], &d, 2);
+  *reinterpret_cast<uint2 *>(o) =
+      make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+}
+
+// Lanes along a row of cells (x), blocks over runs of kDetCellRows cell rows (y) and
+// images (z), so no lane divides.  Walking down its column, a lane keeps the detector
+// row it read for the cell above and loads only the next row: two loads and one vector
+// store per cell, every cell written by exactly one lane.
+constexpr int kDetCellRows = 16;
+
 template <typename TS>
 __global__ void build_det_cells_kernel(const TS *__restrict__ sino, TS *__restrict__ cells,
-                                       int Hd, int Wd, long long total) {
+                                       int Hd, int Wd, int images) {
   const int Wc = Wd + 2, Hc = Hd + 2;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-       t += (long long)gridDim.x * blockDim.x) {
-    long long r = t;
-    const int cu = (int)(r % Wc); r /= Wc;
-    const int cv = (int)(r % Hc);
-    const long long img = r / Hc;
+  const int cu = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cu >= Wc) return;
+  const int u = cu - 1;
+  for (int img = blockIdx.z; img < images; img += gridDim.z) {
     const TS *s = sino + (size_t)img * Hd * Wd;
-    const int u = cu - 1, v = cv - 1;
-    TS c[4];
-    c[0] = pix(s, u, v, Wd, Hd);
-    c[1] = pix(s, u + 1, v, Wd, Hd);
-    c[2] = pix(s, u, v + 1, Wd, Hd);
-    c[3] = pix(s, u + 1, v + 1, Wd, Hd);
-    TS *o = cells + (size_t)t * 4;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) o[q] = c[q];
+    for (int cv0 = blockIdx.y * kDetCellRows; cv0 < Hc; cv0 += gridDim.y * kDetCellRows) {
+      const int cv1 = cv0 + kDetCellRows < Hc ? cv0 + kDetCellRows : Hc;
+      TS *o = cells + (((size_t)img * Hc + cv0) * Wc + cu) * 4;
+      // cell row cv covers detector rows v = cv - 1 and v + 1 = cv
+      TS a = pix(s, u, cv0 - 1, Wd, Hd), b = pix(s, u + 1, cv0 - 1, Wd, Hd);
+#pragma unroll 4
+      for (int cv = cv0; cv < cv1; ++cv, o += (size_t)Wc * 4) {
+        const TS c = pix(s, u, cv, Wd, Hd), d = pix(s, u + 1, cv, Wd, Hd);
+        store_cell(o, a, b, c, d);
+        a = c;
+        b = d;
+      }
+    }
   }
 }
 
@@ -682,11 +706,12 @@ template <typename TS> constexpr int bp_zpt() { return sizeof(TS) == 4 ? 8 : 4; 
 template <typename TS>
 static void launch_det_cells(const void *sino, void *cells, int images, int Hd, int Wd,
                              hipStream_t stream) {
-  const long long total = (long long)images * (Hd + 2) * (Wd + 2);
-  long long blocks = (total + 255) / 256;
-  if (blocks > 65536 * 4) blocks = 65536 * 4;
-  build_det_cells_kernel<TS><<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(
-      (const TS *)sino, (TS *)cells, Hd, Wd, total);
+  const int Hc = Hd + 2;
+  const int runs = (Hc + kDetCellRows - 1) / kDetCellRows;
+  const dim3 grid((unsigned)((Wd + 2 + 255) / 256), (unsigned)(runs < 65535 ? runs : 65535),
+                  (unsigned)(images < 65535 ? images : 65535));
+  build_det_cells_kernel<TS><<<grid, dim3(256), 0, stream>>>((const TS *)sino, (TS *)cells, Hd, Wd,
+                                                             images);
 }
 
 template <typename TS>
@@ -770,6 +795,7 @@ extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, in
     set_error("cone_projection3d: workspace %zu bytes < required %zu", workspace_bytes, need);
     return PYRONN_ERR_WORKSPACE;
   }
+  PYRONN_CHECK_ARG(((uintptr_t)workspace & 15) == 0, "cone_projection3d: workspace must be 16-byte aligned");
   hipStream_t s = (hipStream_t)stream;
   char *ws = (char *)workspace;
   float *inv_ar = (float *)ws;
@@ -839,6 +865,7 @@ extern "C" int pyronn_cone_backprojection3d(const void *sinogram, int sinogram_d
     set_error("cone_backprojection3d: workspace %zu bytes < required %zu", workspace_bytes, need);
     return PYRONN_ERR_WORKSPACE;
   }
+  PYRONN_CHECK_ARG(((uintptr_t)workspace & 15) == 0, "cone_backprojection3d: workspace must be 16-byte aligned");
   hipStream_t s = (hipStream_t)stream;
   const bool hw = hardware_interp != 0;
   const int images = batch * n_views;
@@ -888,6 +915,7 @@ extern "C" int pyronn_cone_backprojection3d_views(
               need);
     return PYRONN_ERR_WORKSPACE;
   }
+  PYRONN_CHECK_ARG(((uintptr_t)workspace & 15) == 0, "cone_backprojection3d_views: workspace must be 16-byte aligned");
   hipStream_t s = (hipStream_t)stream;
   const bool hw = hardware_interp != 0;
   const int images = batch * n_views;
@@ -1050,6 +1078,7 @@ extern "C" int pyronn_cone_projection3d_batched(const pyronn_cone_problem *probl
     set_error("cone_projection3d_batched: workspace %zu bytes < required %zu", workspace_bytes, need);
     return PYRONN_ERR_WORKSPACE;
   }
+  PYRONN_CHECK_ARG(((uintptr_t)workspace & 15) == 0, "cone_projection3d_batched: workspace must be 16-byte aligned");
   hipStream_t s = (hipStream_t)stream;
   const bool hw = hardware_interp != 0;
   char *ws = (char *)workspace;
@@ -1087,6 +1116,7 @@ extern "C" int pyronn_cone_backprojection3d_batched(const pyronn_cone_problem *p
               need);
     return PYRONN_ERR_WORKSPACE;
   }
+  PYRONN_CHECK_ARG(((uintptr_t)workspace & 15) == 0, "cone_backprojection3d_batched: workspace must be 16-byte aligned");
   hipStream_t s = (hipStream_t)stream;
   const bool hw = hardware_interp != 0;
   const int z = z_per_lane > 0 ? z_per_lane : (dtype == PYRONN_F32 ? 8 : 4);

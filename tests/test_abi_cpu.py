@@ -69,6 +69,18 @@ def test_invalid_arguments_are_rejected_before_launch():
     assert rc == 1 and "source" in nat.last_error()
 
 
+def test_workspace_alignment_is_checked():
+    L = nat.lib()
+    need = L.pyronn_cone_projection3d_workspace_size(1, 4, 4, 4, 2, 0)
+    rc = L.pyronn_cone_projection3d(_fake(), 0, 1, 4, 4, 4, _fake(), _fake(), _fake(), _fake(), _fake(), 2, 4, 4, 0,
+                                    _fake(), ctypes.c_void_p(0x1008), need, None)
+    assert rc == 1 and "aligned" in nat.last_error()
+    need = L.pyronn_cone_backprojection3d_workspace_size(1, 2, 4, 4, 0)
+    rc = L.pyronn_cone_backprojection3d(_fake(), 0, 1, 2, 4, 4, _fake(), _fake(), _fake(), _fake(), _fake(), 4, 4, 4,
+                                        0, _fake(), ctypes.c_void_p(0x1004), need, None)
+    assert rc == 1 and "aligned" in nat.last_error()
+
+
 def test_view_range_backprojection_validation():
     L = nat.lib()
     args = lambda mode, ws: (_fake(), 0, 1, 4, 8, 8, _fake(), _fake(), _fake(), _fake(), 4, 4, 4, 0, mode, _fake(),
