@@ -335,6 +335,21 @@ def serve_bench(args):
                 payloads[k].append(noisy.to(storage).cpu().pin_memory())
         plans = {k: (FanFBP(g, storage=storage) if k.startswith("fan") else ConeFDK(g, storage=storage))
                  for k, g in geoms.items()}
+        # the ramp filter alone on the whole 512^3 request (720 views x 768 rows x 1024), resident in HBM:
+        # the fused FFT kernel against the Toeplitz-GEMM filter; bytes = sinogram in + filtered out
+        p512 = plans["cone512"]
+        x = clean["cone512"].to(storage)
+        filt = {}
+        for impl in ("fft", "gemm"):
+            p512.filter_impl = impl
+            filt[impl + "_ms"] = round(_time_events(lambda: p512.filter(x), 5) * 1e3, 3)
+        p512.filter_impl = "fft"
+        nbytes = 2 * x.numel() * x.element_size()
+        filt["fft_GBps"] = round(nbytes / (filt["fft_ms"] * 1e-3) / 1e9, 1)
+        filt["fft_hbm_frac"] = round(filt["fft_GBps"] / 8000.0, 4)
+        log(f"serving {name}: 512^3 ramp filter FFT {filt['fft_ms']:.2f} ms, GEMM {filt['gemm_ms']:.2f} ms")
+        p512.ramp = None
+        del x
         chunked = {"cone512": args.serve_chunks} if args.serve_chunks > 1 else {}
         ex = GpuExecutor(plans, slots=args.serve_slots, use_graphs=True, chunked=chunked)
         log(f"serving {name}: capturing graphs")
@@ -356,6 +371,7 @@ def serve_bench(args):
         log(f"serving {name} saturated: {rep_s['req_per_s']:.2f} req/s")
         out[name]["capacity"] = {k: round(rep_s[k], 3) for k in ("req_per_s", "span_s")}
         out[name]["capacity"]["completed"] = rep_s["completed"]
+        out[name]["filter_cone512"] = filt
         del ex, plans, payloads
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
