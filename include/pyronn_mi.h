@@ -132,13 +132,19 @@ int pyronn_cone_backprojection3d(const void *sinogram, int sinogram_dtype, int b
  * result of pyronn_cone_backprojection3d over all views (mode 2 alone is that operator):
  * every voxel accumulates its views in the same order and the running sum between chunks
  * is the same float the one-pass accumulation holds.
+ * Only the z slices [z_begin, z_end) of the D-slice volume are computed and written (each
+ * voxel's value does not depend on the range), so a caller can split the last chunk
+ * into z slabs and download each slab while the next one is computed.  mode bit 2: the
+ * workspace still holds this chunk's detector cells from the previous call (same
+ * sinogram, same workspace, e.g. the previous z slab); skips rebuilding them.
  */
 int pyronn_cone_backprojection3d_views(const void *sinogram, int sinogram_dtype, int batch,
                                        int n_views, int det_h, int det_w,
                                        const float *volume_origin, const float *volume_spacing,
                                        const float *projection_matrices,
                                        const float *projection_multiplier, int D, int H, int W,
-                                       int hardware_interp, int mode, float *volume,
+                                       int hardware_interp, int mode, int z_begin, int z_end,
+                                       float *volume,
                                        void *workspace, size_t workspace_bytes,
                                        pyronn_stream_t stream);
 

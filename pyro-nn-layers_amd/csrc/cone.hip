@@ -557,6 +557,7 @@ struct BpProblem {
   int W, H, D, n, Hd, Wd, tiles_x, tiles_y, zblocks;
   long long begin;
   int flags;             // kBpContinue: start from the running sums in vol; kBpScale: write sum * mult
+  int z0, z1;            // the z slices [z0, z1) of the volume this launch computes
 };
 constexpr int kBpContinue = 1, kBpScale = 2;
 
@@ -564,8 +565,8 @@ template <typename TS, bool HW, int ZPT>
 __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int zb) {
   const int i = tx * 16 + (threadIdx.x & 15);
   const int j = ty * 16 + (threadIdx.x >> 4);
-  const int k0 = zb * ZPT;
-  const int W = Q.W, H = Q.H, D = Q.D, n = Q.n, Hd = Q.Hd, Wd = Q.Wd;
+  const int k0 = Q.z0 + zb * ZPT;
+  const int W = Q.W, H = Q.H, z1 = Q.z1, n = Q.n, Hd = Q.Hd, Wd = Q.Wd;
   if (i >= W || j >= H) return;
 
   const float sx = Q.spacing[2], sy = Q.spacing[1], sz = Q.spacing[0];
@@ -577,7 +578,7 @@ __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int
     z[q] = fmaf((float)(k0 + q), sz, oz);
     // a running sum continued from an earlier range of views is the same float the
     // accumulation would hold here in one pass, so chunked calls reproduce it exactly
-    acc[q] = ((Q.flags & kBpContinue) && k0 + q < D) ? Q.vol[((size_t)(k0 + q) * H + j) * W + i] : 0.0f;
+    acc[q] = ((Q.flags & kBpContinue) && k0 + q < z1) ? Q.vol[((size_t)(k0 + q) * H + j) * W + i] : 0.0f;
   }
   const int Wc = Wd + 2;
   const size_t img_stride = (size_t)(Hd + 2) * Wc * 4;
@@ -603,7 +604,7 @@ __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int
   const float mu = scale ? Q.mult[0] : 1.0f;
 #pragma unroll
   for (int q = 0; q < ZPT; ++q) {
-    if (k0 + q < D) Q.vol[((size_t)(k0 + q) * H + j) * W + i] = scale ? acc[q] * mu : acc[q];
+    if (k0 + q < z1) Q.vol[((size_t)(k0 + q) * H + j) * W + i] = scale ? acc[q] * mu : acc[q];
   }
 }
 
@@ -717,11 +718,13 @@ static void launch_det_cells(const void *sino, void *cells, int images, int Hd, 
 template <typename TS>
 static void launch_bp(bool hw, const void *cells, float *vol, const float *P, const float *mult,
                       const float *spacing, const float *origin, int B, int D, int H, int W, int n,
-                      int Hd, int Wd, hipStream_t stream, int flags = kBpScale) {
+                      int Hd, int Wd, hipStream_t stream, int flags = kBpScale, int z0 = 0,
+                      int z1 = -1) {
   constexpr int Z = bp_zpt<TS>();
-  const int zblocks = (D + Z - 1) / Z;
+  if (z1 < 0) z1 = D;
+  const int zblocks = (z1 - z0 + Z - 1) / Z;
   BpProblem Q{cells, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd,
-              (W + 15) / 16, (H + 15) / 16, zblocks, 0, flags};
+              (W + 15) / 16, (H + 15) / 16, zblocks, 0, flags, z0, z1};
   dim3 grid((W + 15) / 16, (H + 15) / 16, zblocks * B);
   const size_t cs = det_cells(n, Hd, Wd) * 4;
   if (hw)
@@ -890,18 +893,21 @@ extern "C" int pyronn_cone_backprojection3d_views(
     const void *sinogram, int sinogram_dtype, int batch, int n_views, int det_h, int det_w,
     const float *volume_origin, const float *volume_spacing, const float *projection_matrices,
     const float *projection_multiplier, int D, int H, int W, int hardware_interp, int mode,
-    float *volume, void *workspace, size_t workspace_bytes, pyronn_stream_t stream) {
+    int z_begin, int z_end, float *volume, void *workspace, size_t workspace_bytes,
+    pyronn_stream_t stream) {
   clear_error();
   PYRONN_CHECK_ARG(sinogram && volume_origin && volume_spacing && projection_matrices &&
                        projection_multiplier && volume && workspace,
                    "cone_backprojection3d_views: null pointer");
   PYRONN_CHECK_ARG(dtype_ok(sinogram_dtype), "cone_backprojection3d_views: bad sinogram dtype %d",
                    sinogram_dtype);
-  PYRONN_CHECK_ARG(mode >= 0 && mode <= 3, "cone_backprojection3d_views: bad mode %d", mode);
+  PYRONN_CHECK_ARG(mode >= 0 && mode <= 7, "cone_backprojection3d_views: bad mode %d", mode);
   PYRONN_CHECK_ARG(batch > 0 && D > 0 && H > 0 && W > 0 && n_views > 0 && det_h > 0 && det_w > 0,
                    "cone_backprojection3d_views: bad shape B=%d D=%d H=%d W=%d n=%d det=%dx%d", batch,
                    D, H, W, n_views, det_h, det_w);
-  PYRONN_CHECK_ARG((long long)((D + 3) / 4) * batch <= 65535,
+  PYRONN_CHECK_ARG(z_begin >= 0 && z_begin < z_end && z_end <= D,
+                   "cone_backprojection3d_views: bad z range [%d, %d) for depth %d", z_begin, z_end, D);
+  PYRONN_CHECK_ARG((long long)((z_end - z_begin + 3) / 4) * batch <= 65535,
                    "cone_backprojection3d_views: batch x depth too large for one launch");
   if ((size_t)(det_h + 2) * (det_w + 2) >= (size_t)INT32_MAX / 4 || det_w + 2 >= (1 << 23) ||
       det_h + 2 >= (1 << 23)) {
@@ -921,18 +927,20 @@ extern "C" int pyronn_cone_backprojection3d_views(
   const int images = batch * n_views;
   const int flags = ((mode & 1) ? kBpContinue : 0) | ((mode & 2) ? kBpScale : 0);
   if (sinogram_dtype == PYRONN_F32) {
-    launch_det_cells<float>(sinogram, workspace, images, det_h, det_w, s);
+    if (!(mode & 4)) launch_det_cells<float>(sinogram, workspace, images, det_h, det_w, s);
     launch_bp<float>(hw, workspace, volume, projection_matrices, projection_multiplier,
-                     volume_spacing, volume_origin, batch, D, H, W, n_views, det_h, det_w, s, flags);
+                     volume_spacing, volume_origin, batch, D, H, W, n_views, det_h, det_w, s, flags, z_begin,
+                     z_end);
   } else if (sinogram_dtype == PYRONN_BF16) {
-    launch_det_cells<bf16s>(sinogram, workspace, images, det_h, det_w, s);
+    if (!(mode & 4)) launch_det_cells<bf16s>(sinogram, workspace, images, det_h, det_w, s);
     launch_bp<bf16s>(hw, workspace, volume, projection_matrices, projection_multiplier,
-                     volume_spacing, volume_origin, batch, D, H, W, n_views, det_h, det_w, s, flags);
+                     volume_spacing, volume_origin, batch, D, H, W, n_views, det_h, det_w, s, flags, z_begin,
+                     z_end);
   } else {
-    launch_det_cells<_Float16>(sinogram, workspace, images, det_h, det_w, s);
+    if (!(mode & 4)) launch_det_cells<_Float16>(sinogram, workspace, images, det_h, det_w, s);
     launch_bp<_Float16>(hw, workspace, volume, projection_matrices, projection_multiplier,
                         volume_spacing, volume_origin, batch, D, H, W, n_views, det_h, det_w, s,
-                        flags);
+                        flags, z_begin, z_end);
   }
   PYRONN_CHECK_LAUNCH("cone_bp_kernel");
   return PYRONN_OK;
@@ -1040,7 +1048,7 @@ static int bp_batched(const pyronn_cone_problem *pr, int count, bool hw, int bpc
     const int tx = (p.W + 15) / 16, ty = (p.H + 15) / 16;
     bb.p[k] = BpProblem{cells, p.output, p.projection_matrices, p.projection_multiplier,
                         p.volume_spacing, p.volume_origin, p.W, p.H, p.D, p.n_views, p.det_h,
-                        p.det_w, tx, ty, zblocks, begin, kBpScale};
+                        p.det_w, tx, ty, zblocks, begin, kBpScale, 0, p.D};
     begin += (long long)tx * ty * zblocks;
   }
   bb.count = count;

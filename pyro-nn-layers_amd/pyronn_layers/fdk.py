@@ -154,37 +154,50 @@ class ConeFDK(_Plan):
         return torch.ops.pyronn_layers.cone_backprojection3d(q, list(self.g.volume_shape), self.vo, self.vs,
                                                              self.P, self.step, self.mult, False)
 
-    def backproject_views(self, q: torch.Tensor, view0: int, mode: int, out: torch.Tensor) -> None:
+    def backproject_views(self, q: torch.Tensor, view0: int, mode: int, out: torch.Tensor, z=None,
+                          ws: torch.Tensor | None = None) -> torch.Tensor:
         """Backproject filtered views q [1, k, H_d, W_d] (views view0 .. view0 + k - 1 of the
         orbit) into the running sums in out [1, D, H, W]; mode as
-        pyronn_cone_backprojection3d_views (bit 0 continue, bit 1 apply the FDK scale)."""
+        pyronn_cone_backprojection3d_views (bit 0 continue, bit 1 apply the FDK scale, bit 2
+        reuse the detector cells already in `ws`).  z = (z0, z1) limits the slices computed.
+        Returns the workspace (holding q's detector cells) for a following call with bit 2."""
         _, k, Hd, Wd = q.shape
         D, H, W = self.g.volume_shape
+        z0, z1 = z if z is not None else (0, D)
         code = _DT[q.dtype]
         ws_bytes = nat.lib().pyronn_cone_backprojection3d_workspace_size(1, k, Hd, Wd, code)
-        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=self.device)
+        if ws is None:
+            if mode & 4:
+                raise ValueError("mode bit 2 needs the workspace of the previous call")
+            ws = torch.empty(ws_bytes, dtype=torch.uint8, device=self.device)
         P = self.P[:, view0:view0 + k]
         nat.call("pyronn_cone_backprojection3d_views", _p(q), code, 1, k, Hd, Wd, _p(self.vo), _p(self.vs), _p(P),
-                 _p(self.mult), D, H, W, 0, mode, _p(out), _p(ws), ws_bytes, _stream())
+                 _p(self.mult), D, H, W, 0, mode, z0, z1, _p(out), _p(ws), ws.numel(), _stream())
+        return ws
 
     def chunk_bounds(self, chunks: int):
         n = self.g.n_views
         edges = [round(i * n / chunks) for i in range(chunks + 1)]
         return [(a, b) for a, b in zip(edges[:-1], edges[1:]) if b > a]
 
-    def run_chunked(self, host_sino: torch.Tensor, out: torch.Tensor, bufs, copy_stream, chunks: int = 4):
+    def run_chunked(self, host_sino: torch.Tensor, out: torch.Tensor, bufs, copy_stream, chunks: int = 4,
+                    host_out: torch.Tensor | None = None, out_slabs: int = 4):
         """FDK of a sinogram in pinned host memory [1, n, H_d, W_d] into out [1, D, H, W].
         The views are uploaded in `chunks` ranges on copy_stream, into the two device
         buffers `bufs` (each [1, >= max chunk, H_d, W_d] of the sinogram's dtype) in turn,
         while the current stream weights, filters and backprojects the previous range.
-        The result equals self(device_sinogram) bit for bit (the backprojection continues
-        each voxel's running sum across ranges).  Returns after queueing; synchronize the
-        current stream before reading out."""
+        The result equals self(device_sinogram) bit for bit: the FFT filter treats each
+        view alone and the backprojection continues each voxel's running sum across
+        ranges.  With host_out (pinned, shaped like out) the last range is backprojected in
+        `out_slabs` z slabs, each downloaded on copy_stream while the next is computed.
+        Returns after queueing; synchronize the current stream before reading out or
+        host_out."""
         cur = torch.cuda.current_stream()
         ranges = self.chunk_bounds(chunks)
         free = [torch.cuda.Event(), torch.cuda.Event()]
         for e in free:
             e.record(cur)
+        D = self.g.volume_shape[0]
         for i, (a, b) in enumerate(ranges):
             buf = bufs[i % 2][:, :b - a]
             arrived = torch.cuda.Event()
@@ -194,11 +207,27 @@ class ConeFDK(_Plan):
                 arrived.record(copy_stream)
             cur.wait_event(arrived)
             q = self.filter(buf)
-            mode = (1 if i > 0 else 0) | (2 if i == len(ranges) - 1 else 0)
-            self.backproject_views(q, a, mode, out)
+            last = i == len(ranges) - 1
+            mode = (1 if i > 0 else 0) | (2 if last else 0)
+            if not (last and host_out is not None):
+                self.backproject_views(q, a, mode, out)
+            else:
+                edges = [round(j * D / out_slabs) for j in range(out_slabs + 1)]
+                ws = None
+                for z0, z1 in zip(edges[:-1], edges[1:]):
+                    if z1 <= z0:
+                        continue
+                    ws = self.backproject_views(q, a, mode | (4 if ws is not None else 0), out, (z0, z1), ws)
+                    done = torch.cuda.Event()
+                    done.record(cur)
+                    with torch.cuda.stream(copy_stream):
+                        copy_stream.wait_event(done)
+                        host_out[:, z0:z1].copy_(out[:, z0:z1], non_blocking=True)
+                downloaded = torch.cuda.Event()
+                downloaded.record(copy_stream)
+                cur.wait_event(downloaded)
             free[i % 2] = torch.cuda.Event()
             free[i % 2].record(cur)
-        return out
 
 
 class FanFBP(_Plan):

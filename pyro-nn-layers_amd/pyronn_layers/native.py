@@ -43,7 +43,7 @@ _SIGS = {
     "pyronn_cone_backprojection3d": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i,
                                           _vp, _vp, _sz, _vp]),
     "pyronn_cone_backprojection3d_views": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i,
-                                                _i, _vp, _vp, _sz, _vp]),
+                                                _i, _i, _i, _vp, _vp, _sz, _vp]),
     "pyronn_parallel_projection2d": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _f, _vp,
                                           _vp]),
     "pyronn_parallel_backprojection2d": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp,

@@ -248,8 +248,9 @@ class GpuExecutor:
         st = self.streams[slot]
         with torch.cuda.stream(st):
             if req.kind in self.chunked:
-                p.run_chunked(req.payload, b["out"], b["bufs"], self.copy_streams[slot], self.chunked[req.kind])
-                b["host_out"].copy_(b["out"], non_blocking=True)
+                # the last view range is backprojected in z slabs, each downloaded while the next is computed
+                p.run_chunked(req.payload, b["out"], b["bufs"], self.copy_streams[slot], self.chunked[req.kind],
+                              host_out=b["host_out"])
                 ev = torch.cuda.Event()
                 ev.record(st)
                 return (ev, b)

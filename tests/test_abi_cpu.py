@@ -83,14 +83,17 @@ def test_workspace_alignment_is_checked():
 
 def test_view_range_backprojection_validation():
     L = nat.lib()
-    args = lambda mode, ws: (_fake(), 0, 1, 4, 8, 8, _fake(), _fake(), _fake(), _fake(), 4, 4, 4, 0, mode, _fake(),
-                             _fake(), ws, None)
-    assert L.pyronn_cone_backprojection3d_views(*args(4, 1 << 20)) == 1 and "mode" in nat.last_error()
+    args = lambda mode, ws, z0=0, z1=4: (_fake(), 0, 1, 4, 8, 8, _fake(), _fake(), _fake(), _fake(), 4, 4, 4, 0, mode,
+                                         z0, z1, _fake(), _fake(), ws, None)
+    assert L.pyronn_cone_backprojection3d_views(*args(8, 1 << 20)) == 1 and "mode" in nat.last_error()
     assert L.pyronn_cone_backprojection3d_views(*args(-1, 1 << 20)) == 1
+    for z0, z1 in ((-1, 4), (2, 2), (3, 1), (0, 5)):
+        assert L.pyronn_cone_backprojection3d_views(*args(0, 1 << 20, z0, z1)) == 1
+        assert "z range" in nat.last_error()
     need = L.pyronn_cone_backprojection3d_workspace_size(1, 4, 8, 8, 0)
     assert L.pyronn_cone_backprojection3d_views(*args(3, need - 1)) == 2
     rc = L.pyronn_cone_backprojection3d_views(None, 0, 1, 4, 8, 8, _fake(), _fake(), _fake(), _fake(), 4, 4, 4, 0, 0,
-                                              _fake(), _fake(), need, None)
+                                              0, 4, _fake(), _fake(), need, None)
     assert rc == 1 and "null" in nat.last_error()
 
 

@@ -233,9 +233,23 @@ def test_view_range_backprojection_chains_to_the_one_pass_result(cone_case, stor
             ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
             mode = (1 if i > 0 else 0) | (2 if b == n else 0)
             nat.call("pyronn_cone_backprojection3d_views", p(chunk), code, 1, k, Hd, Wd, p(vo), p(vs),
-                     p(P[:, a:b].contiguous()), p(mu), D, H, W, hw, mode, p(out), p(ws), ws_bytes, stream)
+                     p(P[:, a:b].contiguous()), p(mu), D, H, W, hw, mode, 0, D, p(out), p(ws), ws_bytes, stream)
         torch.cuda.synchronize()
         assert torch.equal(out, ref), edges
+    # the last range in z slabs of uneven depth, reusing its detector cells (mode bit 2)
+    out = torch.full((1, D, H, W), float("nan"), device=dev)
+    for i, (a, b) in enumerate(((0, 45), (45, n))):
+        k = b - a
+        chunk = s[:, a:b].contiguous()
+        ws_bytes = nat.lib().pyronn_cone_backprojection3d_workspace_size(1, k, Hd, Wd, code)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        slabs = [(0, D)] if i == 0 else [(0, 3), (3, 17), (17, 18), (18, D)]
+        for j, (z0, z1) in enumerate(slabs):
+            mode = (1 if i > 0 else 0) | (2 if b == n else 0) | (4 if j > 0 else 0)
+            nat.call("pyronn_cone_backprojection3d_views", p(chunk), code, 1, k, Hd, Wd, p(vo), p(vs),
+                     p(P[:, a:b].contiguous()), p(mu), D, H, W, hw, mode, z0, z1, p(out), p(ws), ws_bytes, stream)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
 
 
 def test_chunked_fdk_from_pinned_host_memory(cone_case):
@@ -255,6 +269,12 @@ def test_chunked_fdk_from_pinned_host_memory(cone_case):
     out = torch.empty((1, D, H, W), device=dev)
     plan.run_chunked(host, out, bufs, torch.cuda.Stream(), chunks=4)
     torch.cuda.synchronize()
+    # the same with the last range in z slabs, each downloaded while the next is computed
+    host_out = torch.full((1, D, H, W), float("nan")).pin_memory()
+    out2 = torch.full((1, D, H, W), float("nan"), device=dev)
+    plan.run_chunked(host, out2, bufs, torch.cuda.Stream(), chunks=4, host_out=host_out, out_slabs=3)
+    torch.cuda.synchronize()
+    assert torch.equal(out2, out) and torch.equal(host_out, out.cpu())
     dsino = host.to(dev)
     q = torch.cat([plan.filter(dsino[:, a:b]) for a, b in ranges], dim=1)
     ref = torch.ops.pyronn_layers.cone_backprojection3d(q, [D, H, W], plan.vo, plan.vs, plan.P, plan.step, plan.mult,
