@@ -20,9 +20,11 @@
 //     the neighbours outside the array read 0.  Weights are exact unless a
 //     caller asks for them quantised to 2^-qbits (the texture unit's 8-bit
 //     fractional weights), in which case an error bound is also produced.
-//   * Every ray march carries an iteration cap, ceil(1.25*len/step) + 16,
-//     that the reference lacks; it only binds when the step is below the
-//     float resolution of alpha, where the reference would never terminate.
+//   * Every ray march carries an iteration cap, ceil(1.25*len/step) + 16 and
+//     at most 2^25, that the reference lacks.  It binds when the step is below
+//     the float resolution of alpha, where the reference would never
+//     terminate, and for rays of more than 2^25 samples (on the largest
+//     volumes the kernels accept, a step below about 0.43 voxel).
 //
 // Templates are instantiated for float (the reference precision) and double
 // (the FP64 twin used to state errors).
@@ -166,11 +168,11 @@ template <class T> inline T interp1(const T *row, int W, T px, int qbits, T *ran
 }
 
 // Iteration cap shared with the HIP kernels.
-// Evaluated in float, exactly as the kernels do; at most 2^22 iterations.
+// Evaluated in float, exactly as the kernels do; at most 2^25 iterations.
 template <class T> inline long long march_cap(T amin, T amax, T step) {
   if (!(amax > amin)) return 0;
   const float n = std::ceil(1.25f * ((float)amax - (float)amin) / (float)step) + 16.0f;
-  return n < 4194304.0f ? (long long)n : 4194304LL;
+  return n < 33554432.0f ? (long long)n : 33554432LL;
 }
 
 // ---------------------------------------------------------------------------

@@ -370,7 +370,8 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   //    of the clip box and its floor indexes an existing cell;
   //  * step >= 2^-20 |alpha|, i.e. at least 8 ulp of alpha: each alpha += step advances by
   //    at least 15/16 step, so the loop ends before 1.07 (amax - amin) / step + 1
-  //    iterations, under the cap of 1.25 (amax - amin) / step + 16 (and under 2^22).
+  //    iterations (at most 2^21 * 1.07 + 1), under the cap of 1.25 (amax - amin) / step + 16
+  //    and under its 2^25 ceiling.
   // The guarded march is used unless every lane of the wave qualifies, so the choice is
   // uniform and never changes a result.
   const float amag = fmaxf(fabsf(amin), fabsf(amax));

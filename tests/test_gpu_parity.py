@@ -125,6 +125,21 @@ def test_cone_forward_guarded_march(case, hw):
     np.testing.assert_array_equal(g, r)
 
 
+@pytest.mark.parametrize("hw", [False, True])
+def test_cone_forward_long_row(hw):
+    """A ray of about 8.4 million samples (more than 2^22) through a 1 x 1 x (2^22 + 1000)
+    volume: the guarded march (the ray is axis-parallel) runs to the end like the oracle's,
+    bit for bit."""
+    from test_oracle_pins import long_row_case
+    vol, P, sp, og = long_row_case()
+    inv, src, _, _, _ = orc.cone_setup(P, sp, og)
+    g = gh.cone_forward(vol, P, sp, og, (3, 3), 0.5, hw=hw)[0]
+    r, _, ns = orc.cone_forward(vol, inv, src, (3, 3), sp, 0.5, hw=hw)
+    assert ns > (1 << 23)
+    np.testing.assert_array_equal(g, r)
+
+
+
 def test_cone_hw_mode_within_texture_weight_bound(cone):
     """hardware_interp against a texture unit's fixed-point filter weights: the HIP result
     (exact weights, HW clip box and sample positions) differs from the oracle with weights

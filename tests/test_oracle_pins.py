@@ -416,3 +416,34 @@ def test_invalid_step_gives_nan_not_a_hang():
     out, _, ns = orc.cone_forward(vol, _axis_ray(0), np.array([[-3.0, 1, 1]], np.float32), (1, 1),
                                   np.ones(3, np.float32), 0.0)
     assert np.isnan(out[0, 0, 0]) and ns == 0
+
+
+def long_row_case(extra=1000):
+    """A 1 x 1 x (2^22 + extra) volume and a 3 x 3 detector whose centre ray runs exactly
+    along x through the whole row: about 8.4 million samples at step 0.5, more than 2^22
+    and with alpha below 2^23, so alpha += 0.5 stays exact (beyond 2^23 it would stall at
+    an even value and the reference's march would never end)."""
+    W = (1 << 22) + extra
+    vol = np.random.default_rng(11).uniform(0.5, 1.0, (1, 1, W)).astype(np.float32)
+    f = 1.0e6
+    K = np.array([[f, 0, 1.0], [0, f, 1.0], [0, 0, 1.0]])
+    R = np.array([[0, 1.0, 0], [0, 0, 1.0], [1.0, 0, 0]])    # camera axes: world y, z, x
+    S = np.array([-100.0, 0.5, 0.5])                          # source (x, y, z), 100 voxels before the row
+    P = np.zeros((1, 3, 4))
+    P[0, :, :3] = K @ R
+    P[0, :, 3] = -(K @ R) @ S
+    return vol, P.astype(np.float32), np.ones(3, np.float32), np.zeros(3, np.float32)
+
+
+def test_cone_forward_long_row_is_not_truncated():
+    """The iteration cap (at most 2^25) does not cut a valid march short: the float march
+    of an 8.4-million-sample ray takes as many samples as its FP64 twin, and its line
+    integral agrees with the FP64 one to the accumulated float rounding."""
+    vol, P, sp, og = long_row_case()
+    inv, src, inv64, src64, st = orc.cone_setup(P, sp, og)
+    assert st == 0
+    r, _, ns = orc.cone_forward(vol, inv, src, (3, 3), sp, 0.5)
+    r64, ns64 = orc.cone_forward_f64(vol, inv64, src64, (3, 3), sp, 0.5)
+    assert ns == ns64 > (1 << 22) + (1 << 22)
+    assert abs(r[0, 1, 1] / r64[0, 1, 1] - 1) < 1e-4
+
