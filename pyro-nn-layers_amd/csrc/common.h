@@ -79,8 +79,8 @@ __device__ __forceinline__ float rcp_rn(float w) { return rcp_fast_ok(w) ? rcp_f
 
 // Iteration cap of every ray march (the reference has none and would spin
 // forever when the step is below the float resolution of alpha).  At most 2^25
-// iterations: with a step of at least half a voxel no ray of any volume the
-// host accepts (W + 2 < 2^23, (H + 2)(D + 3) < 2^23) reaches it.
+// iterations: with a step of at least half a voxel and alpha below 2^23 voxels, no
+// ray of any volume the host accepts (W + 2 < 2^23, (H + 2)(D + 3) < 2^23) reaches it.
 __device__ __forceinline__ int march_cap(float amin, float amax, float step) {
   if (!(amax > amin)) return 0;
   const float n = ceilf(1.25f * (amax - amin) / step) + 16.0f;
