@@ -194,8 +194,8 @@ struct FftPlan {
   }
 };
 
-// LDS word of sequence position p: one pad word per 32, which spreads the strided stores
-// of the early passes over the banks.
+// LDS slot (one complex value, 8 bytes) of sequence position p: one pad slot per 32, which
+// spreads the strided stores of the early passes over the banks.
 __device__ __forceinline__ int pad32(int p) { return p + (p >> 5); }
 
 // Butterfly b (< 8 / R) of a lane in a pass of radix R has index j = lane + b L / 8; its
@@ -239,7 +239,7 @@ __device__ __forceinline__ void fft_twiddle_dft(cpx *v, int lane, const cpx *__r
   }
 }
 
-// Padded LDS word of position base + r * NS.  Linear in r (one address per butterfly,
+// Padded LDS slot of position base + r * NS.  Linear in r (one address per butterfly,
 // immediate offsets) when NS is a multiple of 32 or when R NS <= 32 (then base % 32 + r NS
 // never crosses a 32-word row).
 template <int R, int NS>
@@ -250,7 +250,7 @@ __device__ __forceinline__ int store_word(int base, int r) {
 }
 
 template <int R, int NS, int L>
-__device__ __forceinline__ void fft_store(const cpx *v, float *re, float *im, int lane) {
+__device__ __forceinline__ void fft_store(const cpx *v, cpx *buf, int lane) {
   constexpr int T = L / 8;
 #pragma unroll
   for (int b = 0; b < 8 / R; ++b) {
@@ -258,15 +258,13 @@ __device__ __forceinline__ void fft_store(const cpx *v, float *re, float *im, in
     const int base = (j & ~(NS - 1)) * R + (j & (NS - 1));
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const int a = store_word<R, NS>(base, r);
-      re[a] = v[b * R + r].x;
-      im[a] = v[b * R + r].y;
+      buf[store_word<R, NS>(base, r)] = v[b * R + r];
     }
   }
 }
 
 template <int R, int L>
-__device__ __forceinline__ void fft_load(cpx *v, const float *re, const float *im, int lane) {
+__device__ __forceinline__ void fft_load(cpx *v, const cpx *buf, int lane) {
   constexpr int T = L / 8, S = L / R;
 #pragma unroll
   for (int b = 0; b < 8 / R; ++b) {
@@ -274,7 +272,7 @@ __device__ __forceinline__ void fft_load(cpx *v, const float *re, const float *i
     for (int r = 0; r < R; ++r) {
       const int a = (T % 32 == 0 && S % 32 == 0) ? pad32(lane) + (b * T + r * S) / 32 * 33
                                                  : pad32(lane + b * T + r * S);
-      v[b * R + r] = {re[a], im[a]};
+      v[b * R + r] = buf[a];
     }
   }
 }
@@ -289,16 +287,16 @@ __device__ __forceinline__ int fft_natural(int q, int lane) {
 // Forward passes 1 .. P-1 (pass 0 is done by the caller), each preceded by the LDS
 // transposition out of the previous pass.
 template <int M, int I>
-__device__ __forceinline__ void fft_forward_passes(cpx *v, float *re, float *im, int lane,
+__device__ __forceinline__ void fft_forward_passes(cpx *v, cpx *buf, int lane,
                                                    const cpx *__restrict__ tw) {
   using F = FftPlan<M>;
   if constexpr (I < F::passes) {
     if constexpr (I > 1) __syncthreads();
-    fft_store<F::radix(I - 1), F::ns(I - 1), F::L>(v, re, im, lane);
+    fft_store<F::radix(I - 1), F::ns(I - 1), F::L>(v, buf, lane);
     __syncthreads();
-    fft_load<F::radix(I), F::L>(v, re, im, lane);
+    fft_load<F::radix(I), F::L>(v, buf, lane);
     fft_twiddle_dft<F::radix(I), F::ns(I), F::L>(v, lane, tw);
-    fft_forward_passes<M, I + 1>(v, re, im, lane, tw);
+    fft_forward_passes<M, I + 1>(v, buf, lane, tw);
   }
 }
 
@@ -316,17 +314,17 @@ struct FftInverse {
 };
 
 template <int M, int K>
-__device__ __forceinline__ void fft_inverse_passes(cpx *v, float *re, float *im, int lane,
+__device__ __forceinline__ void fft_inverse_passes(cpx *v, cpx *buf, int lane,
                                                    const cpx *__restrict__ tw) {
   using F = FftPlan<M>;
   using I = FftInverse<M>;
   if constexpr (K < F::passes) {
     __syncthreads();
-    fft_store<I::radix(K - 1), I::ns(K - 1), F::L>(v, re, im, lane);
+    fft_store<I::radix(K - 1), I::ns(K - 1), F::L>(v, buf, lane);
     __syncthreads();
-    fft_load<I::radix(K), F::L>(v, re, im, lane);
+    fft_load<I::radix(K), F::L>(v, buf, lane);
     fft_twiddle_dft<I::radix(K), I::ns(K), F::L>(v, lane, tw);
-    fft_inverse_passes<M, K + 1>(v, re, im, lane, tw);
+    fft_inverse_passes<M, K + 1>(v, buf, lane, tw);
   }
 }
 
@@ -340,7 +338,7 @@ __global__ void __launch_bounds__((1 << M) / 8)
   using F = FftPlan<M>;
   using I = FftInverse<M>;
   constexpr int L = F::L, R0 = F::radix(0), RL = F::radix(F::passes - 1);
-  __shared__ float re[L + L / 32], im[L + L / 32];
+  __shared__ cpx buf[L + L / 32];
   const int lane = threadIdx.x;
   const float *H = spec;
   const cpx *tw = reinterpret_cast<const cpx *>(spec + L);
@@ -368,7 +366,7 @@ __global__ void __launch_bounds__((1 << M) / 8)
     v[q] = {a, b};
   }
   fft_twiddle_dft<R0, 1, L>(v, lane, tw);
-  fft_forward_passes<M, 1>(v, re, im, lane, tw);
+  fft_forward_passes<M, 1>(v, buf, lane, tw);
   // the spectrum is in natural order in v: replace X[f] by conj(H[f] X[f]) (H is real), then
   // run the inverse's first pass (same radix, Ns = 1) on the same values
 #pragma unroll
@@ -377,7 +375,7 @@ __global__ void __launch_bounds__((1 << M) / 8)
     v[q] = {h * v[q].x, -(h * v[q].y)};
   }
   fft_twiddle_dft<I::radix(0), 1, L>(v, lane, tw);
-  fft_inverse_passes<M, 1>(v, re, im, lane, tw);
+  fft_inverse_passes<M, 1>(v, buf, lane, tw);
   // natural order again; the filtered rows are conj(v)
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
