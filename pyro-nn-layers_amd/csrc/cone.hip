@@ -652,6 +652,25 @@ cone_bp_persistent_kernel(BpBatch B, unsigned long long *counters) {
   }
 }
 
+// The same work list as one workgroup per item in item order (no claiming): the hardware
+// dispatches workgroups round-robin over the XCDs in block order, which is the order the
+// single-problem grid has.  The item decode is wave-uniform (scalar), so the kernel keeps
+// the single kernel's register budget; the persistent kernel holds its claim loop
+// around the block and needs more.
+template <typename TS, bool HW, int ZPT>
+__global__ void __launch_bounds__(256) cone_bp_items_kernel(BpBatch B) {
+  const long long w = blockIdx.x;
+  int k = 0;
+  while (k + 1 < B.count && w >= B.p[k + 1].begin) ++k;
+  const BpProblem &Q = B.p[k];
+  const int q = (int)(w - Q.begin);
+  const int per_z = Q.tiles_x * Q.tiles_y;
+  const int zb = q / per_z;
+  const int r = q - zb * per_z;
+  const int ty = r / Q.tiles_x, tx = r - ty * Q.tiles_x;
+  bp_block<TS, HW, ZPT>(Q, tx, ty, zb);
+}
+
 // =====================================================================================
 // Host side
 // =====================================================================================
@@ -1054,6 +1073,13 @@ static int bp_batched(const pyronn_cone_problem *pr, int count, bool hw, int bpc
   }
   bb.count = count;
   bb.total = begin;
+  if (bpc <= 0 && begin < (long long)INT32_MAX) {  // one workgroup per item (the default)
+    if (hw)
+      cone_bp_items_kernel<TS, true, Z><<<dim3((unsigned)begin), dim3(256), 0, s>>>(bb);
+    else
+      cone_bp_items_kernel<TS, false, Z><<<dim3((unsigned)begin), dim3(256), 0, s>>>(bb);
+    return PYRONN_OK;
+  }
   long long grid = (long long)device_cus() * (bpc > 0 ? bpc : 8);
   grid = (grid + kXcds - 1) / kXcds * kXcds;
   reset_counters_kernel<<<1, 64, 0, s>>>(counters);

@@ -2,7 +2,8 @@
 
 Tunable parameters (all preserve results bit for bit):
   forward          blocks_per_cu in {2, 4, 8, 16}  (persistent grid = CUs x value)
-  backprojection   z_per_lane in {4, 8} x blocks_per_cu in {4, 8, 16}
+  backprojection   z_per_lane in {4, 8} x blocks_per_cu in {0, 4, 8, 16}
+                   (0: one workgroup per item; otherwise a persistent grid)
 Each candidate is timed with HIP events (best of `reps` after one warm-up) and
 the winner is cached in a JSON file keyed by the batch signature (shapes,
 views, detector sizes, storage dtype, device name), so a later call with the
@@ -81,7 +82,7 @@ class Autotuner:
         if key not in self.cache:
             trials = {}
             for zpl in (4, 8):
-                for bpc in (4, 8, 16):
+                for bpc in (0, 4, 8, 16):
                     trials[(zpl, bpc)] = _time(lambda: batched.cone_backprojection3d_batched(
                         sinograms, geometries, hardware_interp, z_per_lane=zpl, blocks_per_cu=bpc), self.reps)
             best = min(trials, key=trials.get)

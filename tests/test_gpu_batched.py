@@ -60,7 +60,7 @@ def test_batched_equals_single(storage, hw):
         for s, v, g in zip(sinos, vt, geoms):
             assert torch.equal(s, _single_fwd(v, g, hw))
     st = [s.to(storage) for s in sinos]
-    for zpl, bpc in ((0, 0), (4, 2), (8, 16)):
+    for zpl, bpc in ((0, 0), (4, -1), (4, 2), (8, 16)):   # one workgroup per item, then persistent grids
         recs = batched.cone_backprojection3d_batched(st, geoms, hw, z_per_lane=zpl, blocks_per_cu=bpc)
         for r, s, g in zip(recs, st, geoms):
             assert torch.equal(r, _single_bp(s, g, hw))
