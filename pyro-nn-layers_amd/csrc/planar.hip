@@ -131,19 +131,122 @@ __device__ float march2d(const TS *__restrict__ img, int W, int H, float ox, flo
   return pix;
 }
 
+// Value of lane J of this lane's quad (DPP quad_perm broadcast; all four lanes active).
+template <int J>
+__device__ __forceinline__ float quad_bcast(float f) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(f), J * 0x55, 0xf, 0xf, false));
+}
+
+// The march of march2d with the samples of one ray spread over the G = 4 lanes of a quad:
+// every lane runs the same alpha recurrence (alpha += step, the same float adds), lane q
+// samples positions q, q + G, ... of each group of G, and every lane adds the G values in
+// sample order from quad broadcasts, so all G lanes hold the same running sum, bit for bit
+// the one-lane sum.  The entrance sample and the tail are computed by all lanes alike.
+// Four times the lanes per ray: for launches with too few rays to fill the GPU, where the
+// one-lane march is bound by its own load latency (measured on 256^2 / 180 views x 363
+// pixels: 0.67 -> 0.35 ms; with 261K rays, 2 or 4 lanes per ray were slower than one).
+template <typename TS, bool POW2, int G>
+__device__ float march2d_group(const TS *__restrict__ img, int W, int H, float ox, float oy,
+                               float sx, float sy, float ix, float iy, float s0, float s1, float r0,
+                               float r1, float step, int q) {
+  if (!(step > 0.0f)) return __builtin_nanf("");
+  float amin = 0.0f, amax = __builtin_inff();
+  if (0.0f != r0) {
+    const float e0 = i2p(0.0f, ox, sx) - 0.5f, e1 = i2p((float)W, ox, sx) - 0.5f;
+    const float reci = 1.0f / r0;
+    const float a0 = (e0 - s0) * reci, a1 = (e1 - s0) * reci;
+    amin = fminf(a0, a1);
+    amax = fmaxf(a0, a1);
+  }
+  if (0.0f != r1) {
+    const float e0 = i2p(0.0f, oy, sy) - 0.5f, e1 = i2p((float)H, oy, sy) - 0.5f;
+    const float reci = 1.0f / r1;
+    const float a0 = (e0 - s1) * reci, a1 = (e1 - s1) * reci;
+    amin = fmaxf(amin, fminf(a0, a1));
+    amax = fminf(amax, fmaxf(a0, a1));
+  }
+  float pix = 0.0f, alast = 0.0f;
+  bool any = false;  // a sample was taken: (px, py) of the one-lane march is at alast
+  Tex2dRegs regs;
+  const int cap = march_cap(amin, amax, step);
+#define PYRONN_TEX2D(X, Y) \
+  tex2d_cached(img, W, H, p2i_k<POW2>((X), ox, sx, ix) + 0.5f, p2i_k<POW2>((Y), oy, sy, iy) + 0.5f, regs)
+  if (amin < amax) {
+    pix = fmaf(0.5f, PYRONN_TEX2D(fmaf(amin, r0, s0), fmaf(amin, r1, s1)), pix);
+    alast = amin;
+    any = true;
+    amin += step;
+  }
+  for (int it = 0;;) {
+    float a[G + 1];
+    a[0] = amin;
+#pragma unroll
+    for (int j = 1; j <= G; ++j) a[j] = a[j - 1] + step;
+    // sample j of the group exists iff the one-lane loop would take it: a prefix, as
+    // alpha only grows
+    bool ok[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) ok[j] = (j == 0 || ok[j - 1]) && a[j] < amax && it + j < cap;
+    float my_a = a[0];
+    bool my_ok = ok[0];
+#pragma unroll
+    for (int j = 1; j < G; ++j) {
+      my_a = q == j ? a[j] : my_a;
+      my_ok = q == j ? ok[j] : my_ok;
+    }
+    float f = 0.0f;
+    if (my_ok) f = PYRONN_TEX2D(fmaf(my_a, r0, s0), fmaf(my_a, r1, s1));
+    static_assert(G == 4, "quad broadcasts");
+    const float fg[G] = {quad_bcast<0>(f), quad_bcast<1>(f), quad_bcast<2>(f), quad_bcast<3>(f)};
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      if (ok[j]) pix += fg[j];
+      c += ok[j] ? 1 : 0;
+    }
+    if (c > 0) {
+      alast = a[c - 1];
+      any = true;
+    }
+    amin = a[c];
+    it += c;
+    if (c < G) break;
+  }
+  pix *= step;
+  if (pix > 0.0f) {
+    float px = 0.0f, py = 0.0f;
+    if (any) {
+      px = fmaf(alast, r0, s0);
+      py = fmaf(alast, r1, s1);
+    }
+    const float flast = PYRONN_TEX2D(px, py);
+    pix = fmaf(-(0.5f * step), flast, pix);
+    amin -= step;
+    const float last = amax - amin;
+    pix = fmaf(0.5f * last, flast, pix);
+    px = fmaf(amax, r0, s0);
+    py = fmaf(amax, r1, s1);
+    pix = fmaf(0.5f * last, PYRONN_TEX2D(px, py), pix);
+  }
+#undef PYRONN_TEX2D
+  return pix;
+}
+
 // One lane per (detector pixel, view, batch item).
 // Parallel: project_2Dpar_beam_kernel, par_projector_2D_CudaKernel.cu.cc:104-147.
 // Fan:      project_2Dfan_beam_kernel, fan_projector_2D_CudaKernel.cu.cc:104-149.
-template <typename TS, bool FAN>
+// G lanes per pixel (1, or 4 with march2d_group for launches with few rays).
+template <typename TS, bool FAN, int G>
 __global__ void __launch_bounds__(256)
 proj2d_kernel(const TS *__restrict__ vol, float *__restrict__ sino, const float *__restrict__ rays,
               const float *__restrict__ vorigin, const float *__restrict__ dorigin,
               const float *__restrict__ vspacing, const float *__restrict__ dspacing,
               const float *__restrict__ sid_p, const float *__restrict__ sdd_p, int H, int W, int n,
               int det, float step_factor) {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = t / G, q = t % G;
   const int view = blockIdx.y, b = blockIdx.z;
-  if (d >= det) return;
+  if (d >= det) return;  // the G lanes of a pixel share d
   const float sx = vspacing[b * 2 + 1], sy = vspacing[b * 2 + 0];
   const float ox = vorigin[b * 2 + 1], oy = vorigin[b * 2 + 0];
   const size_t vb = (size_t)b * n + view;
@@ -173,11 +276,17 @@ proj2d_kernel(const TS *__restrict__ vol, float *__restrict__ sino, const float 
   const TS *img = vol + (size_t)b * H * W;
   // spacings are per batch item, so the choice is uniform over the block
   const float ix = pow2_reciprocal(sx), iy = pow2_reciprocal(sy);
-  const float pix = (ix != 0.0f && iy != 0.0f)
-                        ? march2d<TS, true>(img, W, H, ox, oy, sx, sy, ix, iy, s0, s1, r0, r1, step)
-                        : march2d<TS, false>(img, W, H, ox, oy, sx, sy, ix, iy, s0, s1, r0, r1, step);
+  float pix;
+  if constexpr (G > 1)
+    pix = (ix != 0.0f && iy != 0.0f)
+              ? march2d_group<TS, true, G>(img, W, H, ox, oy, sx, sy, ix, iy, s0, s1, r0, r1, step, q)
+              : march2d_group<TS, false, G>(img, W, H, ox, oy, sx, sy, ix, iy, s0, s1, r0, r1, step, q);
+  else
+    pix = (ix != 0.0f && iy != 0.0f)
+              ? march2d<TS, true>(img, W, H, ox, oy, sx, sy, ix, iy, s0, s1, r0, r1, step)
+              : march2d<TS, false>(img, W, H, ox, oy, sx, sy, ix, iy, s0, s1, r0, r1, step);
   const float a = r0 * sx, c = r1 * sy;
-  sino[vb * det + d] = pix * sqrtf(fmaf(c, c, a * a));
+  if (q == 0) sino[vb * det + d] = pix * sqrtf(fmaf(c, c, a * a));
 }
 
 // 2D line intersection, helper_geometry_gpu.h:28-44 (NaN when |den| < 1e-6).
@@ -264,29 +373,50 @@ back2d_kernel(const TS *__restrict__ sino, float *__restrict__ vol, const float 
   }
 }
 
+// rays per launch below which a ray gets four lanes
+constexpr long long kQuadRayLimit = 131072;
+
+template <bool FAN, int G>
+static void launch_proj2d_g(dim3 grid, const void *volume, int dtype, int H, int W, const float *vo,
+                            const float *dor, const float *vs, const float *ds, const float *sid,
+                            const float *sdd, const float *rays, int n, int det, float step_factor,
+                            float *sino, hipStream_t s) {
+  if (dtype == PYRONN_F32)
+    proj2d_kernel<float, FAN, G><<<grid, dim3(256), 0, s>>>((const float *)volume, sino, rays, vo, dor, vs,
+                                                            ds, sid, sdd, H, W, n, det, step_factor);
+  else if (dtype == PYRONN_BF16)
+    proj2d_kernel<bf16s, FAN, G><<<grid, dim3(256), 0, s>>>((const bf16s *)volume, sino, rays, vo, dor, vs,
+                                                            ds, sid, sdd, H, W, n, det, step_factor);
+  else
+    proj2d_kernel<_Float16, FAN, G><<<grid, dim3(256), 0, s>>>(
+        (const _Float16 *)volume, sino, rays, vo, dor, vs, ds, sid, sdd, H, W, n, det, step_factor);
+}
+
 template <bool FAN>
 static int launch_proj2d(const char *name, const void *volume, int dtype, int batch, int H, int W,
                          const float *vo, const float *dor, const float *vs, const float *ds,
-                         const float *sid, const float *sdd, const float *rays, int n, int det,
+                         const float *sid, const float *sdd, const float *rays_p, int n, int det,
                          float step_factor, float *sino, hipStream_t s) {
   clear_error();
-  PYRONN_CHECK_ARG(volume && vo && dor && vs && ds && rays && sino, "%s: null pointer", name);
+  PYRONN_CHECK_ARG(volume && vo && dor && vs && ds && rays_p && sino, "%s: null pointer", name);
   PYRONN_CHECK_ARG(!FAN || (sid && sdd), "%s: null source distances", name);
   PYRONN_CHECK_ARG(dtype_ok(dtype), "%s: bad dtype %d", name, dtype);
   PYRONN_CHECK_ARG(batch > 0 && batch <= 65535 && H > 0 && W > 0 && n > 0 && n <= 65535 && det > 0,
                    "%s: bad shape B=%d H=%d W=%d n=%d det=%d", name, batch, H, W, n, det);
   PYRONN_CHECK_ARG((long long)H * W < INT32_MAX, "%s: image too large", name);
   PYRONN_CHECK_ARG(step_factor > 0.0f, "%s: step factor must be > 0", name);
-  dim3 grid((det + 255) / 256, n, batch);
-  if (dtype == PYRONN_F32)
-    proj2d_kernel<float, FAN><<<grid, dim3(256), 0, s>>>((const float *)volume, sino, rays, vo, dor, vs,
-                                                         ds, sid, sdd, H, W, n, det, step_factor);
-  else if (dtype == PYRONN_BF16)
-    proj2d_kernel<bf16s, FAN><<<grid, dim3(256), 0, s>>>((const bf16s *)volume, sino, rays, vo, dor, vs,
-                                                         ds, sid, sdd, H, W, n, det, step_factor);
+  PYRONN_CHECK_ARG(det <= INT32_MAX / 4, "%s: too many detector pixels", name);
+  // Few rays (one lane each would leave most SIMDs with a handful of latency-bound waves):
+  // four lanes per ray.  Both variants give the same bits.
+  const long long nrays = (long long)det * n * batch;
+  const int G = nrays < kQuadRayLimit ? 4 : 1;
+  const dim3 grid((unsigned)(((long long)G * det + 255) / 256), n, batch);
+  if (G == 4)
+    launch_proj2d_g<FAN, 4>(grid, volume, dtype, H, W, vo, dor, vs, ds, sid, sdd, rays_p, n, det, step_factor,
+                            sino, s);
   else
-    proj2d_kernel<_Float16, FAN><<<grid, dim3(256), 0, s>>>(
-        (const _Float16 *)volume, sino, rays, vo, dor, vs, ds, sid, sdd, H, W, n, det, step_factor);
+    launch_proj2d_g<FAN, 1>(grid, volume, dtype, H, W, vo, dor, vs, ds, sid, sdd, rays_p, n, det, step_factor,
+                            sino, s);
   PYRONN_CHECK_LAUNCH(name);
   return PYRONN_OK;
 }

@@ -380,3 +380,32 @@ def test_2d_non_power_of_two_spacings_and_mixed_batch(fan_beam):
         rb, _ = bwd_o(r, rays, (H, W), orgs[k], spacings[k], dos[k], dss[k], **okw)
         l2, mx, exact = gh.parity(gb, rb)
         assert exact == 1.0, (k, exact)
+
+
+@pytest.mark.parametrize("fan_beam", [False, True])
+def test_2d_forward_one_and_four_lane_marches(fan_beam):
+    """Launches with fewer than 2^17 rays spread each ray's samples over four lanes
+    (march2d_group); larger launches march one lane per ray.  The same image and geometry
+    through both (a batch of one, and batches of four and eight) must match the oracle bit
+    for bit."""
+    H, W = 40, 36
+    s = np.array([0.5, 0.5], np.float32)
+    org = geo.centred_origin((H, W), s).astype(np.float32)
+    e = ph.random_ellipses(71, 8, (H, W), s, axes_range_px=(3, 16))
+    img = ph.rasterise_ellipses(e, (H, W), s, org)
+    n, det = 720, 100                                   # 72,000 rays per image
+    rays = (geo.fan_central_ray_vectors(n, 2 * math.pi) if fan_beam else geo.parallel_ray_vectors(n)).astype(
+        np.float32)
+    ds, do = np.float32(0.35), np.float32(-(det - 1) / 2 * 0.35)
+    kw = dict(sid=np.float32(60.0), sdd=np.float32(90.0)) if fan_beam else {}
+    fwd_o = orc.fan_forward if fan_beam else orc.parallel_forward
+    r, _, _ = fwd_o(img, rays, det, org, s, do, ds, step_factor=0.2, **kw)
+    one = gh.forward2d(fan_beam, img, rays, det, org, s, do, ds, step_factor=0.2, **kw)[0]
+    np.testing.assert_array_equal(one, r)
+    for B in (4, 8):                                    # 288,000 and 576,000 rays: one lane per ray
+        many = gh.forward2d(fan_beam, np.stack([img] * B), np.stack([rays] * B), det, np.stack([org] * B),
+                            np.stack([s] * B), np.full(B, do, np.float32), np.full(B, ds, np.float32),
+                            step_factor=0.2, **kw)
+        for b in range(B):
+            np.testing.assert_array_equal(many[b], r)
+
