@@ -409,3 +409,28 @@ def test_2d_forward_one_and_four_lane_marches(fan_beam):
         for b in range(B):
             np.testing.assert_array_equal(many[b], r)
 
+
+@pytest.mark.parametrize("beam", ["par", "fan"])
+def test_2d_within_texture_weight_bound(beam, par, fan):
+    """The reference's 2D ops sample through the texture unit (8-bit fixed-point weights);
+    the HIP sampler uses exact weights.  Forward and backprojection must differ from the
+    oracle with weights quantised to 1/256 by no more than the per-element bound that
+    oracle computes (forward: 2 * 2^-9 * bound, backprojection: 2^-9 * bound)."""
+    c = par if beam == "par" else fan
+    fb = beam == "fan"
+    kw = dict(sid=c["sid"], sdd=c["sdd"]) if fb else {}
+    f = orc.fan_forward if fb else orc.parallel_forward
+    b = orc.fan_backward if fb else orc.parallel_backward
+    g = gh.forward2d(fb, c["img"], c["rays"], c["det"], c["org"], c["s"], c["do"], c["ds"], **kw)[0]
+    q, bnd, _ = f(c["img"], c["rays"], c["det"], c["org"], c["s"], c["do"], c["ds"], qbits=8, want_bound=True, **kw)
+    d = np.abs(g - q)
+    print(f"{beam} fwd vs 8-bit weights: {int((d > 0).sum())} of {d.size} differ, max {d.max():.3e}")
+    assert d.max() > 0
+    assert np.all(d <= 2 * 2.0 ** -9 * bnd + 1e-5)
+    sino = np.ascontiguousarray(g)
+    gb = gh.backward2d(fb, sino, c["rays"], c["shape"], c["org"], c["s"], c["do"], c["ds"], **kw)[0]
+    qb, bb = b(sino, c["rays"], c["shape"], c["org"], c["s"], c["do"], c["ds"], qbits=8, want_bound=True, **kw)
+    d = np.abs(gb - qb)
+    print(f"{beam} bp vs 8-bit weights: {int((d > 0).sum())} of {d.size} differ, max {d.max():.3e}")
+    assert np.all(d <= 2.0 ** -9 * bb + 1e-5)
+
