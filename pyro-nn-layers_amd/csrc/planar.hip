@@ -76,15 +76,13 @@ __device__ __forceinline__ float tex1d(const TS *__restrict__ row, int W, float 
   return lerp_ref(p0, p1, bx - fx);
 }
 
-// kernel_project2D, par_projector_2D_CudaKernel.cu.cc:28-102 (same code in
-// fan_projector_2D_CudaKernel.cu.cc:27-102): march in mm, box edges with the
-// -0.5 mm offset, first sample exactly at the entry point, tail for a positive sum.
-template <typename TS, bool POW2>
-__device__ float march2d(const TS *__restrict__ img, int W, int H, float ox, float oy, float sx,
-                         float sy, float ix, float iy, float s0, float s1, float r0, float r1,
-                         float step) {
-  if (!(step > 0.0f)) return __builtin_nanf("");
-  float amin = 0.0f, amax = __builtin_inff();
+// Ray parameter interval inside the image box, whose edges sit 0.5 mm before each index
+// edge (par_projector_2D_CudaKernel.cu.cc:33-57); an axis-parallel ray skips that slab.
+__device__ __forceinline__ void clip2d(int W, int H, float ox, float oy, float sx, float sy,
+                                       float s0, float s1, float r0, float r1, float &amin,
+                                       float &amax) {
+  amin = 0.0f;
+  amax = __builtin_inff();
   if (0.0f != r0) {
     const float e0 = i2p(0.0f, ox, sx) - 0.5f, e1 = i2p((float)W, ox, sx) - 0.5f;
     const float reci = 1.0f / r0;
@@ -99,6 +97,18 @@ __device__ float march2d(const TS *__restrict__ img, int W, int H, float ox, flo
     amin = fmaxf(amin, fminf(a0, a1));
     amax = fminf(amax, fmaxf(a0, a1));
   }
+}
+
+// kernel_project2D, par_projector_2D_CudaKernel.cu.cc:28-102 (same code in
+// fan_projector_2D_CudaKernel.cu.cc:27-102): march in mm, box edges with the
+// -0.5 mm offset, first sample exactly at the entry point, tail for a positive sum.
+template <typename TS, bool POW2>
+__device__ float march2d(const TS *__restrict__ img, int W, int H, float ox, float oy, float sx,
+                         float sy, float ix, float iy, float s0, float s1, float r0, float r1,
+                         float step) {
+  if (!(step > 0.0f)) return __builtin_nanf("");
+  float amin, amax;
+  clip2d(W, H, ox, oy, sx, sy, s0, s1, r0, r1, amin, amax);
   float pix = 0.0f, px = 0.0f, py = 0.0f;
   Tex2dRegs regs;
   const int cap = march_cap(amin, amax, step);
@@ -150,21 +160,8 @@ __device__ float march2d_group(const TS *__restrict__ img, int W, int H, float o
                                float sx, float sy, float ix, float iy, float s0, float s1, float r0,
                                float r1, float step, int q) {
   if (!(step > 0.0f)) return __builtin_nanf("");
-  float amin = 0.0f, amax = __builtin_inff();
-  if (0.0f != r0) {
-    const float e0 = i2p(0.0f, ox, sx) - 0.5f, e1 = i2p((float)W, ox, sx) - 0.5f;
-    const float reci = 1.0f / r0;
-    const float a0 = (e0 - s0) * reci, a1 = (e1 - s0) * reci;
-    amin = fminf(a0, a1);
-    amax = fmaxf(a0, a1);
-  }
-  if (0.0f != r1) {
-    const float e0 = i2p(0.0f, oy, sy) - 0.5f, e1 = i2p((float)H, oy, sy) - 0.5f;
-    const float reci = 1.0f / r1;
-    const float a0 = (e0 - s1) * reci, a1 = (e1 - s1) * reci;
-    amin = fmaxf(amin, fminf(a0, a1));
-    amax = fminf(amax, fmaxf(a0, a1));
-  }
+  float amin, amax;
+  clip2d(W, H, ox, oy, sx, sy, s0, s1, r0, r1, amin, amax);
   float pix = 0.0f, alast = 0.0f;
   bool any = false;  // a sample was taken: (px, py) of the one-lane march is at alast
   Tex2dRegs regs;
