@@ -288,8 +288,10 @@ int pyronn_fdk_filter(const void *sinogram, int sinogram_dtype, int images, int 
  * pyronn_fdk_filter_fft: filtered[i][r][:] = ((weights[r][:] .* sinogram[i][r][:])
  * conv h) .* column_weights, stored as filtered_dtype; no workspace.  Rows 2p and
  * 2p + 1 of an image are filtered together (as one complex sequence), so a row's
- * result depends only on its own image, whatever the number of images per call.
- * Results agree with the GEMM filter to float rounding (about 1e-7 relative). */
+ * result depends only on its own image, whatever the number of images per call; a NaN or
+ * infinity in one of the two rows makes both filtered rows non-finite (the GEMM filter:
+ * only its own row).  Results agree with the GEMM filter to float rounding (about 1e-7
+ * relative). */
 size_t pyronn_fdk_ramp_spectrum_size(int det_w);
 int pyronn_fdk_ramp_spectrum(int det_w, double tau, const float *column_weights, float *spectrum,
                              pyronn_stream_t stream);
