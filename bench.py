@@ -53,8 +53,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-storage run")
     ap.add_argument("--no-hw", action="store_true", help="skip the hardware_interp run")
-    ap.add_argument("--cpu-views", type=int, default=4, help="forward views in the CPU sample")
-    ap.add_argument("--cpu-slices", type=int, default=16, help="z slices in the CPU backprojection sample")
+    ap.add_argument("--cpu-views", type=int, default=12, help="forward views in the CPU sample")
+    ap.add_argument("--cpu-slices", type=int, default=48, help="z slices in the CPU backprojection sample")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 1 GiB device copy (reference for HBM byte counters)")
     ap.add_argument("--bf16-only", action="store_true", help="time only the BF16-storage run")
