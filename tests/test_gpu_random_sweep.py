@@ -1,8 +1,8 @@
 """Seeded random-geometry parity sweep: cone projection / backprojection with arbitrary
 projection matrices (random camera rotations, principal points and focal lengths, sources
 outside and inside the volume), anisotropic spacings, offset origins, odd and tiny shapes,
-several step sizes, both interpolation modes and BF16 storage; 2D parallel / fan with
-random view angles, spacings (power of two or not) and step factors.  Every output must
+several step sizes, both interpolation modes and BF16/FP16 storage; 2D parallel / fan with
+random view angles, spacings (power of two or not), step factors and BF16 storage.  Every output must
 equal the oracle's bit for bit: the kernels and the oracle perform the same IEEE
 operations in the same order, whatever the geometry."""
 import math
@@ -66,16 +66,21 @@ def test_cone_random_geometry_bit_exact(seed):
         g = gh.cone_forward(vol, P, sp, og, det, step, hw=hw)[0]
         r, _, _ = orc.cone_forward(vol, inv, src, det, sp, step, hw=hw)
         assert np.array_equal(g, r), (seed, hw, shape, det, step, int((g != r).sum()))
-    vq = gh.quantise(vol, "bf16")
-    g = gh.cone_forward(vol, P, sp, og, det, step, storage="bf16")[0]
-    r, _, _ = orc.cone_forward(vq, inv, src, det, sp, step)
-    assert np.array_equal(g, r), (seed, "bf16", int((g != r).sum()))
+    for storage in ("bf16", "f16"):      # 16-bit storage: the oracle on the values the kernel reads
+        vq = gh.quantise(vol, storage)
+        g = gh.cone_forward(vol, P, sp, og, det, step, storage=storage)[0]
+        r, _, _ = orc.cone_forward(vq, inv, src, det, sp, step)
+        assert np.array_equal(g, r), (seed, storage, int((g != r).sum()))
     rng = np.random.default_rng(1000 + seed)
     sino = rng.uniform(-1, 2, (P.shape[0],) + det).astype(np.float32)
     for hw in (False, True):
         gb = gh.cone_backward(sino, P, sp, og, shape, mult=0.7, hw=hw)[0]
         rb, _ = orc.cone_backward(sino, P, shape, sp, og, mult=0.7, hw=hw)
         assert np.array_equal(gb, rb), (seed, "bp", hw, int((gb != rb).sum()))
+    sq = gh.quantise(sino, "bf16")
+    gb = gh.cone_backward(sino, P, sp, og, shape, mult=0.7, storage="bf16")[0]
+    rb, _ = orc.cone_backward(sq, P, shape, sp, og, mult=0.7)
+    assert np.array_equal(gb, rb), (seed, "bp bf16", int((gb != rb).sum()))
 
 
 @pytest.mark.parametrize("seed", range(16))
@@ -106,4 +111,11 @@ def test_2d_random_geometry_bit_exact(seed, fan_beam):
     sino = rng.uniform(-1, 2, (n, det)).astype(np.float32)
     gb = gh.backward2d(fan_beam, sino, rays, (H, W), org, s, do, ds, **kw)[0]
     rb, _ = bwd_o(sino, rays, (H, W), org, s, do, ds, **okw)
+    np.testing.assert_array_equal(gb, rb)
+    # bf16 storage: the oracle on the values the kernels read
+    g = gh.forward2d(fan_beam, img, rays, det, org, s, do, ds, step_factor=step, storage="bf16", **kw)[0]
+    r, _, _ = fwd_o(gh.quantise(img, "bf16"), rays, det, org, s, do, ds, step_factor=step, **okw)
+    np.testing.assert_array_equal(g, r)
+    gb = gh.backward2d(fan_beam, sino, rays, (H, W), org, s, do, ds, storage="bf16", **kw)[0]
+    rb, _ = bwd_o(gh.quantise(sino, "bf16"), rays, (H, W), org, s, do, ds, **okw)
     np.testing.assert_array_equal(gb, rb)
