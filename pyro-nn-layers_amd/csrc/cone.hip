@@ -513,7 +513,11 @@ __global__ void build_det_cells_kernel(const TS *__restrict__ sino, TS *__restri
 
 // Bilinear sample, lerp x then y (cone_backprojector_3D_CudaKernel.cu.cc:55-82),
 // zero outside the detector.
-template <typename TS, bool HW>
+// FIDX: img points at cell (0, 0) (not at the cell array's start, cell (-1, -1)) and the
+// host guarantees (H_d + 2)(W_d + 2) <= 2^24, so the cell offset fy W_c + fx is an integer
+// of magnitude below 2^24 that one float FMA forms exactly (two instructions fewer than
+// the integer form).
+template <typename TS, bool HW, bool FIDX>
 __device__ __forceinline__ float sample_det(const TS *__restrict__ img, int Wc, float u, float v,
                                             float fW, float fH) {
   if (HW) {  // tex2DLayered(u + 0.5, v + 0.5, n)
@@ -525,7 +529,7 @@ __device__ __forceinline__ float sample_det(const TS *__restrict__ img, int Wc, 
   const float fx = floorf(u), fy = floorf(v);
   const float xd = u - fx, yd = v - fy;
   // signed 24-bit multiply (full rate); the host keeps W_d + 2 and H_d + 2 below 2^23
-  const unsigned c = (unsigned)(__mul24((int)fy, Wc) + (int)fx + (Wc + 1));
+  const int c = FIDX ? (int)fmaf(fy, (float)Wc, fx) : __mul24((int)fy, Wc) + (int)fx + (Wc + 1);
   float p00, p01, p10, p11;
   if constexpr (sizeof(TS) == 4) {
     const float4 a = reinterpret_cast<const float4 *>(img)[c];
@@ -562,7 +566,7 @@ struct BpProblem {
 };
 constexpr int kBpContinue = 1, kBpScale = 2;
 
-template <typename TS, bool HW, int ZPT>
+template <typename TS, bool HW, int ZPT, bool FIDX>
 __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int zb) {
   const int i = tx * 16 + (threadIdx.x & 15);
   const int j = ty * 16 + (threadIdx.x >> 4);
@@ -583,7 +587,7 @@ __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int
   }
   const int Wc = Wd + 2;
   const size_t img_stride = (size_t)(Hd + 2) * Wc * 4;
-  const TS *img = static_cast<const TS *>(Q.cells);
+  const TS *img = static_cast<const TS *>(Q.cells) + (FIDX ? (size_t)(Wc + 1) * 4 : 0);
   const float *Pm = Q.P;
   const float fW = (float)Wd, fH = (float)Hd;
 
@@ -597,7 +601,7 @@ __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int
       const float ww = rcp_rn(fmaf(m[10], z[q], tw) + m[11]);  // correctly rounded, as the reference's division
       const float uu = (fmaf(m[2], z[q], tu) + m[3]) * ww;
       const float vv = (fmaf(m[6], z[q], tv) + m[7]) * ww;
-      const float f = sample_det<TS, HW>(img, Wc, uu, vv, fW, fH);
+      const float f = sample_det<TS, HW, FIDX>(img, Wc, uu, vv, fW, fH);
       acc[q] = fmaf(f * ww, ww, acc[q]);
     }
   }
@@ -609,7 +613,7 @@ __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int
   }
 }
 
-template <typename TS, bool HW, int ZPT>
+template <typename TS, bool HW, int ZPT, bool FIDX>
 __global__ void __launch_bounds__(256) cone_bp_kernel(BpProblem Q, size_t cells_stride) {
   const int b = blockIdx.z / Q.zblocks;
   const int zb = blockIdx.z - b * Q.zblocks;
@@ -619,7 +623,7 @@ __global__ void __launch_bounds__(256) cone_bp_kernel(BpProblem Q, size_t cells_
   Q.mult += b;
   Q.spacing += (size_t)b * 3;
   Q.origin += (size_t)b * 3;
-  bp_block<TS, HW, ZPT>(Q, blockIdx.x, blockIdx.y, zb);
+  bp_block<TS, HW, ZPT, FIDX>(Q, blockIdx.x, blockIdx.y, zb);
 }
 
 struct BpBatch {
@@ -633,7 +637,7 @@ struct BpBatch {
 // so at any time all XCDs work on neighbouring tiles of the same z block and share
 // that block's band of detector rows (of every view) in the Infinity Cache - the
 // order of the single-problem grid.
-template <typename TS, bool HW, int ZPT>
+template <typename TS, bool HW, int ZPT, bool FIDX>
 __global__ void __launch_bounds__(256)
 cone_bp_persistent_kernel(BpBatch B, unsigned long long *counters) {
   const int xcd = (int)(blockIdx.x % kXcds);
@@ -648,7 +652,7 @@ cone_bp_persistent_kernel(BpBatch B, unsigned long long *counters) {
     const int zb = (int)(q / per_z);
     const long long r = q - (long long)zb * per_z;
     const int ty = (int)(r / Q.tiles_x), tx = (int)(r - (long long)ty * Q.tiles_x);
-    bp_block<TS, HW, ZPT>(Q, tx, ty, zb);
+    bp_block<TS, HW, ZPT, FIDX>(Q, tx, ty, zb);
   }
 }
 
@@ -657,7 +661,7 @@ cone_bp_persistent_kernel(BpBatch B, unsigned long long *counters) {
 // single-problem grid has.  The item decode is wave-uniform (scalar), so the kernel keeps
 // the single kernel's register budget; the persistent kernel holds its claim loop
 // around the block and needs more.
-template <typename TS, bool HW, int ZPT>
+template <typename TS, bool HW, int ZPT, bool FIDX>
 __global__ void __launch_bounds__(256) cone_bp_items_kernel(BpBatch B) {
   const long long w = blockIdx.x;
   int k = 0;
@@ -668,7 +672,7 @@ __global__ void __launch_bounds__(256) cone_bp_items_kernel(BpBatch B) {
   const int zb = q / per_z;
   const int r = q - zb * per_z;
   const int ty = r / Q.tiles_x, tx = r - ty * Q.tiles_x;
-  bp_block<TS, HW, ZPT>(Q, tx, ty, zb);
+  bp_block<TS, HW, ZPT, FIDX>(Q, tx, ty, zb);
 }
 
 // =====================================================================================
@@ -686,6 +690,8 @@ static bool fwd_volume_fits(int dtype, int D, int H, int W) {
          (long long)(H + 2) * (D + zx) < (1LL << 23);
 }
 static size_t det_cells(int n, int Hd, int Wd) { return (size_t)n * (Hd + 2) * (Wd + 2); }
+// Whether the backprojection may form detector cell offsets with one float FMA (exact below 2^24).
+static bool det_fidx(int Hd, int Wd) { return (long long)(Hd + 2) * (Wd + 2) <= (1LL << 24); }
 
 // Blocks of one forward launch per batch item: 8 XCD lanes x tile-row bands x views x tiles.
 static long long fwd_blocks(int n, int Hd, int Wd) {
@@ -747,10 +753,17 @@ static void launch_bp(bool hw, const void *cells, float *vol, const float *P, co
               (W + 15) / 16, (H + 15) / 16, zblocks, 0, flags, z0, z1};
   dim3 grid((W + 15) / 16, (H + 15) / 16, zblocks * B);
   const size_t cs = det_cells(n, Hd, Wd) * 4;
-  if (hw)
-    cone_bp_kernel<TS, true, Z><<<grid, dim3(256), 0, stream>>>(Q, cs);
-  else
-    cone_bp_kernel<TS, false, Z><<<grid, dim3(256), 0, stream>>>(Q, cs);
+  if (det_fidx(Hd, Wd)) {
+    if (hw)
+      cone_bp_kernel<TS, true, Z, true><<<grid, dim3(256), 0, stream>>>(Q, cs);
+    else
+      cone_bp_kernel<TS, false, Z, true><<<grid, dim3(256), 0, stream>>>(Q, cs);
+  } else {
+    if (hw)
+      cone_bp_kernel<TS, true, Z, false><<<grid, dim3(256), 0, stream>>>(Q, cs);
+    else
+      cone_bp_kernel<TS, false, Z, false><<<grid, dim3(256), 0, stream>>>(Q, cs);
+  }
 }
 
 }  // namespace pyronn
@@ -1073,20 +1086,30 @@ static int bp_batched(const pyronn_cone_problem *pr, int count, bool hw, int bpc
   }
   bb.count = count;
   bb.total = begin;
+  bool fidx = true;
+  for (int k = 0; k < count; ++k) fidx = fidx && det_fidx(pr[k].det_h, pr[k].det_w);
   if (bpc <= 0 && begin < (long long)INT32_MAX) {  // one workgroup per item (the default)
-    if (hw)
-      cone_bp_items_kernel<TS, true, Z><<<dim3((unsigned)begin), dim3(256), 0, s>>>(bb);
-    else
-      cone_bp_items_kernel<TS, false, Z><<<dim3((unsigned)begin), dim3(256), 0, s>>>(bb);
+    const dim3 g((unsigned)begin);
+    if (fidx) {
+      if (hw) cone_bp_items_kernel<TS, true, Z, true><<<g, dim3(256), 0, s>>>(bb);
+      else cone_bp_items_kernel<TS, false, Z, true><<<g, dim3(256), 0, s>>>(bb);
+    } else {
+      if (hw) cone_bp_items_kernel<TS, true, Z, false><<<g, dim3(256), 0, s>>>(bb);
+      else cone_bp_items_kernel<TS, false, Z, false><<<g, dim3(256), 0, s>>>(bb);
+    }
     return PYRONN_OK;
   }
   long long grid = (long long)device_cus() * (bpc > 0 ? bpc : 8);
   grid = (grid + kXcds - 1) / kXcds * kXcds;
   reset_counters_kernel<<<1, 64, 0, s>>>(counters);
-  if (hw)
-    cone_bp_persistent_kernel<TS, true, Z><<<dim3((unsigned)grid), dim3(256), 0, s>>>(bb, counters);
-  else
-    cone_bp_persistent_kernel<TS, false, Z><<<dim3((unsigned)grid), dim3(256), 0, s>>>(bb, counters);
+  const dim3 g((unsigned)grid);
+  if (fidx) {
+    if (hw) cone_bp_persistent_kernel<TS, true, Z, true><<<g, dim3(256), 0, s>>>(bb, counters);
+    else cone_bp_persistent_kernel<TS, false, Z, true><<<g, dim3(256), 0, s>>>(bb, counters);
+  } else {
+    if (hw) cone_bp_persistent_kernel<TS, true, Z, false><<<g, dim3(256), 0, s>>>(bb, counters);
+    else cone_bp_persistent_kernel<TS, false, Z, false><<<g, dim3(256), 0, s>>>(bb, counters);
+  }
   return PYRONN_OK;
 }
 
