@@ -166,10 +166,10 @@ def test_cone_hw_mode_within_texture_weight_bound(cone):
 def test_cone_backprojection_edge_quirk_report(cone):
     """The reference's bilinear bounds test (x > W_d, y > H_d) reads the first pixel of
     the next detector row (or the next view) for samples past the last column (row);
-    the build reads zero there.  With a detector smaller than the volume's shadow some
+    this library reads zero there.  With a detector smaller than the volume's shadow some
     voxels sample that edge: report how many differ from the oracle's emulation of the
-    reference reads and by how much (reported, not graded), and check that every other
-    voxel is identical."""
+    reference reads and by how much (a reported difference, not a failure), and check that
+    every other voxel is identical."""
     Hd, Wd = 30, 34
     P = geo.circular_projection_matrices(12, 150.0, 250.0, (Hd, Wd), (1.4, 1.3), start=0.1).astype(np.float32)
     rng = np.random.default_rng(6)
