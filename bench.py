@@ -69,6 +69,8 @@ def main(argv=None):
     ap.add_argument("--batched-only", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the C1-C3 configuration measurements")
     ap.add_argument("--configs-only", action="store_true")
+    ap.add_argument("--filter-only", action="store_true",
+                    help="time the FDK ramp filter (FFT and GEMM) on one 512^3 request's sinogram")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the library has no multi-GPU path.",
@@ -91,6 +93,19 @@ def main(argv=None):
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    # the single-section modes need neither the 512^3 phantom nor its sample count
+    if args.serve_only:
+        print(json.dumps(serve_bench(args)), flush=True)
+        return 0
+    if args.batched_only:
+        print(json.dumps(batched_bench(args)), flush=True)
+        return 0
+    if args.configs_only:
+        print(json.dumps(configs_bench(args)), flush=True)
+        return 0
+    if args.filter_only:
+        print(json.dumps(filter_bench(args)), flush=True)
+        return 0
     cfg = m1_config()
     D, H, W = cfg["shape"]
     n = cfg["n"]
@@ -142,15 +157,6 @@ def main(argv=None):
         return float(np.mean(times_f)), float(np.mean(times_b)), sino
 
     # Timed region: K steps bracketed by synchronize on both sides (inside run()).
-    if args.serve_only:
-        print(json.dumps(serve_bench(args)), flush=True)
-        return 0
-    if args.batched_only:
-        print(json.dumps(batched_bench(args)), flush=True)
-        return 0
-    if args.configs_only:
-        print(json.dumps(configs_bench(args)), flush=True)
-        return 0
     if args.bf16_only:
         tf, tb, _ = run(torch.bfloat16)
         log(f"bf16 only: fwd {tf * 1e3:.1f} ms, bp {tb * 1e3:.1f} ms")
@@ -375,6 +381,36 @@ def serve_bench(args):
         del ex, plans, payloads
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
+    return out
+
+
+def filter_bench(args):
+    """The FDK ramp filter alone on the sinogram of one 512^3 request (720 views x 768 x
+    1024, seeded uniform values, resident in HBM): the fused FFT kernel and the
+    Toeplitz-GEMM filter, FP32 and BF16 storage; bytes = sinogram in + filtered out."""
+    import torch
+
+    from pyronn_layers import geometry as geo
+    from pyronn_layers.fdk import ConeFDK
+
+    dev = torch.device("cuda:0")
+    g = geo.ConeGeometry((512, 512, 512), (1.0, 1.0, 1.0), (768, 1024), (1.2, 1.2), 720, 750.0, 1200.0)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(7)
+    sino = torch.rand((1, g.n_views) + tuple(g.detector_shape), device=dev, generator=gen)
+    out = {"workload": "cone 512^3 request: 720 views x 768 x 1024 detector rows"}
+    for name, storage in (("f32", torch.float32), ("bf16", torch.bfloat16)):
+        plan = ConeFDK(g, storage=storage)
+        x = sino.to(storage)
+        res = {}
+        for impl in ("fft", "gemm"):
+            plan.filter_impl = impl
+            res[impl + "_ms"] = round(_time_events(lambda: plan.filter(x), 5) * 1e3, 3)
+        nbytes = 2 * x.numel() * x.element_size()
+        res["fft_GBps"] = round(nbytes / (res["fft_ms"] * 1e-3) / 1e9, 1)
+        log(f"filter {name}: FFT {res['fft_ms']:.2f} ms, GEMM {res['gemm_ms']:.2f} ms")
+        out[name] = res
+        del plan, x
     return out
 
 
