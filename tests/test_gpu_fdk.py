@@ -252,6 +252,35 @@ def test_view_range_backprojection_chains_to_the_one_pass_result(cone_case, stor
     assert torch.equal(out, ref)
 
 
+def test_view_range_z_slabs_with_a_batch_of_two(cone_case):
+    """The view-range entry point with batch 2 (two sinograms, two geometries): z slabs of
+    uneven depth, the later ones reusing the detector cells, give each item's one-pass
+    backprojection bit for bit."""
+    import ctypes
+    from pyronn_layers import native as nat
+    g, _, sino = cone_case
+    dev = torch.device("cuda")
+    n, (Hd, Wd), (D, H, W) = g.n_views, g.detector_shape, g.volume_shape
+    g2 = geo.ConeGeometry(g.volume_shape, g.volume_spacing, g.detector_shape, g.detector_spacing, n, 170.0, 300.0)
+    s = torch.stack([torch.tensor(sino, device=dev), torch.tensor(sino[::-1].copy(), device=dev) * 0.5])
+    P = torch.tensor(np.stack([g.projection_matrices(), g2.projection_matrices()]), dtype=torch.float32, device=dev)
+    vo = torch.tensor(np.stack([g.volume_origin, g2.volume_origin]), dtype=torch.float32, device=dev)
+    vs = torch.tensor(np.stack([g.volume_spacing, g2.volume_spacing]), dtype=torch.float32, device=dev)
+    mu = torch.tensor([[0.37], [1.3]], device=dev)
+    ref = torch.ops.pyronn_layers.cone_backprojection3d(s, [D, H, W], vo, vs, P, torch.full((2, 1), 0.5, device=dev),
+                                                        mu, False)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    ws_bytes = nat.lib().pyronn_cone_backprojection3d_workspace_size(2, n, Hd, Wd, nat.F32)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    out = torch.full((2, D, H, W), float("nan"), device=dev)
+    for j, (z0, z1) in enumerate(((0, 5), (5, 6), (6, 23), (23, D))):
+        nat.call("pyronn_cone_backprojection3d_views", p(s), nat.F32, 2, n, Hd, Wd, p(vo), p(vs), p(P), p(mu), D, H, W,
+                 0, 2 | (4 if j else 0), z0, z1, p(out), p(ws), ws_bytes, stream)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+
+
 def test_chunked_fdk_from_pinned_host_memory(cone_case):
     """ConeFDK.run_chunked uploads view ranges on a second stream while the first filters
     and backprojects: its result equals filtering each range and backprojecting all views
