@@ -1,6 +1,6 @@
 """Throughput of the cone-beam projector / backprojector pair on one MI355X.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3]
 
 Workload (the headline configuration): cone-beam circular orbit, volume 512^3
 float32 at 1 mm with a centred origin, 720 views over 2 pi, 1024 x 768 detector
@@ -48,11 +48,12 @@ def m1_config():
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-storage run")
     ap.add_argument("--no-hw", action="store_true", help="skip the hardware_interp run")
+    ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage run")
     ap.add_argument("--cpu-views", type=int, default=12, help="forward views in the CPU sample")
     ap.add_argument("--cpu-slices", type=int, default=48, help="z slices in the CPU backprojection sample")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -130,6 +131,8 @@ def main(argv=None):
     R = n * Hd * Wd
     V = D * H * W
 
+    medians = {}
+
     def run(storage, hw=False):
         v = vol.to(storage)
         times_f, times_b = [], []
@@ -154,6 +157,7 @@ def main(argv=None):
                 times_f.append(tf)
                 times_b.append(tb)
             del rec
+        medians[(storage, hw)] = (float(np.median(times_f)), float(np.median(times_b)))
         return float(np.mean(times_f)), float(np.mean(times_b)), sino
 
     # Timed region: K steps bracketed by synchronize on both sides (inside run()).
@@ -212,6 +216,8 @@ def main(argv=None):
         },
         "fwd_back": {
             "t_fwd_ms": round(tf * 1e3, 2), "t_bp_ms": round(tb * 1e3, 2),
+            "t_fwd_median_ms": round(medians[(torch.float32, False)][0] * 1e3, 2),
+            "t_bp_median_ms": round(medians[(torch.float32, False)][1] * 1e3, 2),
             "S_fwd": S_fwd, "U_bp": U_bp,
             "fwd_gsamples_s": round(S_fwd / tf / 1e9, 2), "bp_gupdates_s": round(U_bp / tb / 1e9, 2),
             "algorithmic_bytes": B_fwd + B_bp,
@@ -236,6 +242,20 @@ def main(argv=None):
         }
     else:
         del sino
+
+    if not args.no_f16:
+        torch.cuda.empty_cache()
+        tfh16, tbh16, sino_f16 = run(torch.float16)
+        del sino_f16
+        Bf = S_fwd * 8 * 2 + R * 4
+        Bb = U_bp * 4 * 2 + V * 4
+        gh16 = (S_fwd + U_bp) / (tfh16 + tbh16) / 1e9
+        out["f16"] = {
+            "value": round(gh16, 3), "unit": "Gsamples/s", "speedup_vs_f32": round(gh16 / gs, 3),
+            "t_fwd_ms": round(tfh16 * 1e3, 2), "t_bp_ms": round(tbh16 * 1e3, 2),
+            "hbm_roofline_frac_fwd_back": round((Bf + Bb) / (tfh16 + tbh16) / HBM_PEAK, 4),
+            "storage": "fp16 volume and sinogram, fp32 accumulation, fp32 outputs",
+        }
 
     if not args.no_hw:
         # hardware_interp: texture clip box [-0.5, N-0.5] and sample positions, exact weights

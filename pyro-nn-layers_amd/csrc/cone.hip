@@ -11,8 +11,9 @@
 //
 // The forward march and the backprojection sum follow the reference kernels
 // operation by operation (see the citations at each kernel), so that results
-// agree with the host restatement to the last bit in almost every element.
+// agree with the host restatement to the last bit.
 #include "common.h"
+#include <type_traits>
 
 namespace pyronn {
 
@@ -66,23 +67,28 @@ __global__ void cone_setup_kernel(const float *__restrict__ P, const float *__re
 // z-fastest so that the 64 rays of a wave (8 detector rows ~ 8 z positions, and z
 // is transverse to every ray of an orbit about the z axis) fall into few cache lines
 // for every view direction.  The zero border makes the reference's per-neighbour
-// bounds tests unnecessary.  Two layouts, chosen by storage type (measured on the
-// 512^3 / 720-view workload):
-//  * f32 - xy quads: cell (x, y, z), x in [-1, W], y in [-1, H], z in [-1, D + 1],
-//    holds [v(x,y,z), v(x+1,y,z), v(x,y+1,z), v(x+1,y+1,z)] (16 bytes) at index
-//    ((x+1) (H+2) + (y+1)) (D+3) + (z+1).  A sample reads cells c and c + 1 (the z
-//    and z+1 quads), two adjacent 16-byte loads sharing cache lines.
-//  * bf16 / f16 - 2x2x2 cells: cell (x, y, z), x, y, z in [-1, N], holds the quad at
-//    z followed by the quad at z + 1 (16 bytes) at ((x+1) (H+2) + (y+1)) (D+2) + (z+1).
-//    A sample is one 16-byte load.
+// bounds tests unnecessary.  Layout (xy quads): cell (x, y, z), x in [-1, W],
+// y in [-1, H], z in [-1, D + 1], holds [v(x,y,z), v(x+1,y,z), v(x,y+1,z), v(x+1,y+1,z)]
+// at index ((x+1) (H+2) + (y+1)) (D+3) + (z+1).  A sample needs the quads at z and
+// z + 1, which are adjacent:
+//  * f32 - 16-byte quads; a sample is two adjacent 16-byte loads sharing cache lines;
+//  * bf16 / f16 - 8-byte quads; a sample is one 16-byte load at 8-byte alignment.
+// Every storage type thus spends one quad per z step of a column: the 16-bit copy is
+// half the bytes of the f32 one.  (A 2x2x2 cell per z step for 16-bit storage, also one
+// 16-byte load per sample, occupies as many bytes as the f32 quads: on the 512^3 /
+// 720-view forward it took 473 ms against 389 ms for these quads in bf16, as the 16-bit
+// gathers are limited by L2 misses rather than by instruction issue.)
 // The build goes through a z-fastest transpose of the volume so that both of its
 // passes read and write contiguously.
 // =====================================================================================
 template <typename TS> struct CellLayout {
-  static constexpr bool kQuad = sizeof(TS) == 4;  // f32: xy quads; 16-bit: 2x2x2 cells
-  static constexpr int kPerCell = kQuad ? 4 : 8;  // elements per cell
-  static constexpr int kZExtra = kQuad ? 3 : 2;   // cells along z beyond D
+  static constexpr int kPerCell = 4;  // elements per cell (an xy quad)
+  static constexpr int kZExtra = 3;   // cells along z beyond D
 };
+
+// The 16 bytes of the z and z + 1 quads of a 16-bit cell array: 8-byte aligned (the
+// target supports unaligned vector loads, so this stays one global_load_dwordx4).
+typedef uint32_t u32x4_a8 __attribute__((ext_vector_type(4), aligned(8)));
 
 template <typename TS> static size_t cell_elems(int D, int H, int W) {
   return (size_t)(W + 2) * (H + 2) * (D + CellLayout<TS>::kZExtra) * CellLayout<TS>::kPerCell;
@@ -134,17 +140,11 @@ __global__ void build_cells_kernel(const TS *__restrict__ vt, TS *__restrict__ c
     const long long b = r / Wc;
     const TS *v = vt + (size_t)b * W * H * D;
     const int x = cx - 1, y = cy - 1, z = cz - 1;
-    TS c[8];
+    TS c[4];
     c[0] = voxt(v, x, y, z, W, H, D);
     c[1] = voxt(v, x + 1, y, z, W, H, D);
     c[2] = voxt(v, x, y + 1, z, W, H, D);
     c[3] = voxt(v, x + 1, y + 1, z, W, H, D);
-    if (!L::kQuad) {
-      c[4] = voxt(v, x, y, z + 1, W, H, D);
-      c[5] = voxt(v, x + 1, y, z + 1, W, H, D);
-      c[6] = voxt(v, x, y + 1, z + 1, W, H, D);
-      c[7] = voxt(v, x + 1, y + 1, z + 1, W, H, D);
-    }
     TS *o = cells + (size_t)t * L::kPerCell;
 #pragma unroll
     for (int q = 0; q < L::kPerCell; ++q) o[q] = c[q];
@@ -167,9 +167,23 @@ __device__ __forceinline__ void unpack2(uint32_t w, _Float16, float &a, float &b
 // those reuse the registers instead of loading again.  The blend is the same
 // arithmetic on the same values either way.  2^32 - 1 is never a cell index (the
 // host keeps the cell count below 2^31), so the first sample of a ray always loads.
-struct CellRegs {
+// f32 and bf16 keep the corners as floats (bf16 widened once per load).  f16 keeps the
+// packed cell: the blend's FMAs read the halves directly (v_fma_mix_f32 widens an f16
+// operand exactly inside the FMA, one rounding as before), so a load costs no
+// conversions at all.
+template <typename TS> struct CellRegs {
   unsigned cell = 0xffffffffu;
   float c[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  __device__ __forceinline__ float corner(int i) const { return c[i]; }
+};
+template <> struct CellRegs<_Float16> {
+  unsigned cell = 0xffffffffu;
+  uint32_t w[4] = {0u, 0u, 0u, 0u};  // corners 2k (low half) and 2k + 1 (high half)
+  __device__ __forceinline__ float corner(int i) const {
+    _Float16 h[2];
+    __builtin_memcpy(h, &w[i >> 1], 4);
+    return (float)h[i & 1];
+  }
 };
 
 // Trilinear sample at (px, py, pz), lerp z -> y -> x exactly as
@@ -183,7 +197,7 @@ struct CellRegs {
 template <typename TS, bool CLAMP>
 __device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int Dc, int HDc,
                                               float px, float py, float pz, float fW, float fH,
-                                              float fD, CellRegs &r) {
+                                              float fD, CellRegs<TS> &r) {
   if (CLAMP) {
     px = clamp_apron(px, fW);
     py = clamp_apron(py, fH);
@@ -195,24 +209,29 @@ __device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int 
   const unsigned cell = (unsigned)(__mul24((int)fx, HDc) + yz + (HDc + Dc + 1));
   if (cell != r.cell) {
     r.cell = cell;
-    if constexpr (CellLayout<TS>::kQuad) {
+    if constexpr (sizeof(TS) == 4) {
       const float4 *c4 = reinterpret_cast<const float4 *>(cells) + cell;
       const float4 a = c4[0], b = c4[1];  // quads at z and z + 1
       r.c[0] = a.x; r.c[1] = a.y; r.c[2] = a.z; r.c[3] = a.w;
       r.c[4] = b.x; r.c[5] = b.y; r.c[6] = b.z; r.c[7] = b.w;
     } else {
-      const uint4 q = reinterpret_cast<const uint4 *>(cells)[cell];
-      unpack2(q.x, TS{}, r.c[0], r.c[1]);
-      unpack2(q.y, TS{}, r.c[2], r.c[3]);
-      unpack2(q.z, TS{}, r.c[4], r.c[5]);
-      unpack2(q.w, TS{}, r.c[6], r.c[7]);
+      // quads at z and z + 1: words 0, 1 and 2, 3
+      const u32x4_a8 q = *reinterpret_cast<const u32x4_a8 *>(reinterpret_cast<const uint2 *>(cells) + cell);
+      if constexpr (std::is_same<TS, _Float16>::value) {
+        r.w[0] = q.x; r.w[1] = q.y; r.w[2] = q.z; r.w[3] = q.w;
+      } else {
+        unpack2(q.x, TS{}, r.c[0], r.c[1]);
+        unpack2(q.y, TS{}, r.c[2], r.c[3]);
+        unpack2(q.z, TS{}, r.c[4], r.c[5]);
+        unpack2(q.w, TS{}, r.c[6], r.c[7]);
+      }
     }
   }
-  // r.c = p000 p001 p010 p011 (z) then the same four at z + 1; digits are z, y, x.
-  const float p00 = lerp_ref(r.c[0], r.c[4], zd);
-  const float p01 = lerp_ref(r.c[1], r.c[5], zd);
-  const float p10 = lerp_ref(r.c[2], r.c[6], zd);
-  const float p11 = lerp_ref(r.c[3], r.c[7], zd);
+  // corners: p000 p001 p010 p011 (z) then the same four at z + 1; digits are z, y, x.
+  const float p00 = lerp_ref(r.corner(0), r.corner(4), zd);
+  const float p01 = lerp_ref(r.corner(1), r.corner(5), zd);
+  const float p10 = lerp_ref(r.corner(2), r.corner(6), zd);
+  const float p11 = lerp_ref(r.corner(3), r.corner(7), zd);
   const float p0 = lerp_ref(p00, p10, yd);
   const float p1 = lerp_ref(p01, p11, yd);
   return lerp_ref(p0, p1, xd);
@@ -221,7 +240,7 @@ __device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int 
 template <typename TS, bool HW, bool CLAMP>
 __device__ __forceinline__ float cone_sample(const TS *__restrict__ c, int Dc, int HDc, float px,
                                              float py, float pz, float fW, float fH, float fD,
-                                             CellRegs &r) {
+                                             CellRegs<TS> &r) {
   if (HW) {  // tex3D(p + 0.5): the sampler subtracts the half texel again
     px = (px + 0.5f) - 0.5f;
     py = (py + 0.5f) - 0.5f;
@@ -240,7 +259,7 @@ __device__ __forceinline__ float march_ray(const TS *__restrict__ v, int Dc, int
                                            float s0, float s1, float s2, float amin, float amax,
                                            float step) {
   float pix = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
-  CellRegs regs;
+  CellRegs<TS> regs;
   const int cap = GUARDED ? march_cap(amin, amax, step) : 0;
   if (amin < amax) {
     px = fmaf(amin, rx, s0);

@@ -120,9 +120,12 @@ def test_workspace_sizes_scale_with_storage():
     # per view: (H_d + 2) x (W_d + 2) cells of 4 neighbours
     assert L.pyronn_cone_backprojection3d_workspace_size(2, 10, 8, 8, 0) == \
         ((2 * 10 * 10 * 10 * 4 * 4 + 255) // 256) * 256
-    # geometry (12 floats per view) + cells ((N+2)^3 x 8 neighbours) + z-fastest transpose of the volume
+    # geometry (12 floats per view) + cells ((W+2)(H+2)(D+3) xy quads) + z-fastest transpose of the volume
     assert L.pyronn_cone_projection3d_workspace_size(1, 4, 4, 4, 2, 1) == \
-        ((1 * 2 * 12 * 4 + 255) // 256) * 256 + ((6 * 6 * 6 * 8 * 2 + 255) // 256) * 256 + 256
+        ((1 * 2 * 12 * 4 + 255) // 256) * 256 + ((6 * 6 * 7 * 4 * 2 + 255) // 256) * 256 + 256
+    # the 16-bit copy is half the bytes of the f32 one
+    assert L.pyronn_cone_projection3d_workspace_size(1, 4, 4, 4, 2, 0) == \
+        ((1 * 2 * 12 * 4 + 255) // 256) * 256 + ((6 * 6 * 7 * 4 * 4 + 255) // 256) * 256 + 256
 
 
 def test_torch_ops_registered_and_cpu_tensors_rejected():
