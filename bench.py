@@ -209,9 +209,9 @@ def main(argv=None):
             "traffic_source": "profiles/cone_m1_f32_traffic.json (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
             # measured HBM bytes over the live duration: the share of HBM bandwidth actually used
             "traffic_frac": round(traffic / t_dom / HBM_PEAK, 4) if traffic else None,
-            "limiter": ("vector-memory address path (texture-address unit 98.6% busy, one L1 access per sample; "
+            "limiter": ("vector-memory address path (texture-address unit 99.9% busy, one L1 access per sample; "
                         "profiles/cone_m1_counters.md): algorithmic bytes exceed HBM peak because the L1 serves "
-                        "~92% of corner reads")
+                        "~92% of corner reads and the L2 ~80% of the rest")
             if dominant == "cone_fwd_kernel" else None,
         },
         "fwd_back": {

@@ -299,10 +299,14 @@ __device__ __forceinline__ float march_ray(const TS *__restrict__ v, int Dc, int
 // than 8 x 8 waves at 512^3).
 // Work order: workgroups are dispatched round-robin over the 8 XCDs, so block id
 // b runs on XCD b % 8.  XCD k takes detector tile rows k, k+8, k+16, ...; within a
-// tile row it walks views (outer) and tiles along the row (inner).  The blocks
-// resident on one XCD at any time are therefore a few neighbouring views of one
-// 16-row band of the detector: their rays cross the same z slab of the volume,
-// nearly coincide from view to view, and share that XCD's L2.  The order changes
+// tile row it walks tiles (outer) and views (inner).  The blocks resident on one XCD
+// at any time are therefore one or two 16 x 16 tiles of one 16-row band over a few
+// hundred consecutive views: their rays cross the same z slab of the volume, and the
+// rays of one pixel in consecutive views are half a degree apart, so lines fetched
+// for one view are read again by the next while they are still in that XCD's L2.
+// (Measured on the 512^3 / 720-view forward against views outer, tiles inner: 483
+// against 534 ms in f32, 328 against 390 ms in bf16; groups of 2 to 32 tiles per pass
+// over the views fell in between, the fewer tiles the faster.)  The order changes
 // only which lane computes which ray, never the arithmetic of a ray.
 // Ray setup: cone_projector_3D_CudaKernel.cu.cc:197-221; march: :87-181
 // (hardware_interp: cone_projector_3D_CudaKernel_hardware_interp.cu.cc:43-142).
@@ -329,8 +333,8 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   const int ty = band * kXcds + xcd;
   if (ty >= P.tiles_y) return;
   const long long w_ = qq - (long long)band * per_row;
-  const int view = (int)(w_ / P.tiles_x);
-  const int tx = (int)(w_ - (long long)view * P.tiles_x);
+  const int tx = (int)(w_ / P.n);
+  const int view = (int)(w_ - (long long)tx * P.n);
   const int col = tx * 16 + wave * 4 + (lane & 3);
   const int row = ty * 16 + (lane >> 2);
   const int W = P.W, H = P.H, D = P.D, Hd = P.Hd, Wd = P.Wd;
