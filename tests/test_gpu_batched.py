@@ -47,7 +47,7 @@ def _single_bp(s, g, hw):
         torch.full((1, 1), g.step_size, device=dev), torch.full((1, 1), g.projection_multiplier, device=dev), hw)[0]
 
 
-@pytest.mark.parametrize("storage", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("storage", [torch.float32, torch.bfloat16, torch.float16])
 @pytest.mark.parametrize("hw", [False, True])
 def test_batched_equals_single(storage, hw):
     import pyronn_layers as pl
