@@ -64,8 +64,8 @@ __global__ void cone_setup_kernel(const float *__restrict__ P, const float *__re
 // =====================================================================================
 // Neighbourhood cells.  The forward projector reads the volume through a copy made
 // of cells that hold a voxel neighbourhood, with zeros outside the volume, stored
-// z-fastest so that the 64 rays of a wave (8 detector rows ~ 8 z positions, and z
-// is transverse to every ray of an orbit about the z axis) fall into few cache lines
+// z-fastest so that the 64 rays of a wave (16 detector rows ~ 16 neighbouring z
+// positions, and z is transverse to every ray of an orbit about the z axis) fall into few cache lines
 // for every view direction.  The zero border makes the reference's per-neighbour
 // bounds tests unnecessary.  Layout (xy quads): cell (x, y, z), x in [-1, W],
 // y in [-1, H], z in [-1, D + 1], holds [v(x,y,z), v(x+1,y,z), v(x,y+1,z), v(x+1,y+1,z)]
