@@ -20,6 +20,8 @@ torch = pytest.importorskip("torch")
 def _cuda():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
+    import pyronn_layers as pl
+    pl.load()   # registers torch.ops.pyronn_layers, which some tests call directly
 
 
 @pytest.fixture(scope="module")
