@@ -588,6 +588,8 @@ struct BpProblem {
   int z0, z1;            // the z slices [z0, z1) of the volume this launch computes
 };
 constexpr int kBpContinue = 1, kBpScale = 2;
+constexpr int kBpRegionsX = 4, kBpRegionsY = 2;  // regions of a z block's tiles, one per XCD
+static_assert(kBpRegionsX * kBpRegionsY == kXcds, "one region per XCD");
 
 template <typename TS, bool HW, int ZPT, bool FIDX>
 __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int zb) {
@@ -646,7 +648,18 @@ __global__ void __launch_bounds__(256) cone_bp_kernel(BpProblem Q, size_t cells_
   Q.mult += b;
   Q.spacing += (size_t)b * 3;
   Q.origin += (size_t)b * 3;
-  bp_block<TS, HW, ZPT, FIDX>(Q, blockIdx.x, blockIdx.y, zb);
+  // XCD-compact tiles: the (x, y) tiles of a z block form a 4 x 2 grid of regions and XCD
+  // k (block id mod 8) takes region k, row by row, so the blocks resident on an XCD are
+  // neighbours whose detector footprints overlap in its L2 (f32 72.6 against 76.1 ms with
+  // consecutive tiles dealt round-robin over the XCDs, on the 512^3 / 720-view
+  // backprojection; 8 x 1, 2 x 4 and 1 x 8 region grids were in between or slower).  The
+  // grid is padded to 8 equal regions; padding blocks exit.
+  const int rw = (Q.tiles_x + kBpRegionsX - 1) / kBpRegionsX;
+  const int rh = (Q.tiles_y + kBpRegionsY - 1) / kBpRegionsY;
+  const int xcd = (int)(blockIdx.x % kXcds), j = (int)(blockIdx.x / kXcds);
+  const int tx = (xcd % kBpRegionsX) * rw + j % rw, ty = (xcd / kBpRegionsX) * rh + j / rw;
+  if (tx >= Q.tiles_x || ty >= Q.tiles_y) return;
+  bp_block<TS, HW, ZPT, FIDX>(Q, tx, ty, zb);
 }
 
 struct BpBatch {
@@ -774,7 +787,9 @@ static void launch_bp(bool hw, const void *cells, float *vol, const float *P, co
   const int zblocks = (z1 - z0 + Z - 1) / Z;
   BpProblem Q{cells, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd,
               (W + 15) / 16, (H + 15) / 16, zblocks, 0, flags, z0, z1};
-  dim3 grid((W + 15) / 16, (H + 15) / 16, zblocks * B);
+  const int rw = ((W + 15) / 16 + kBpRegionsX - 1) / kBpRegionsX;
+  const int rh = ((H + 15) / 16 + kBpRegionsY - 1) / kBpRegionsY;
+  dim3 grid(kXcds * rw * rh, 1, zblocks * B);
   const size_t cs = det_cells(n, Hd, Wd) * 4;
   if (det_fidx(Hd, Wd)) {
     if (hw)
