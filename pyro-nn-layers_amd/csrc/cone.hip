@@ -588,8 +588,29 @@ struct BpProblem {
   int z0, z1;            // the z slices [z0, z1) of the volume this launch computes
 };
 constexpr int kBpContinue = 1, kBpScale = 2;
-constexpr int kBpRegionsX = 4, kBpRegionsY = 2;  // regions of a z block's tiles, one per XCD
+// XCD-compact tiles: the (x, y) tiles of a z block form a 4 x 2 grid of regions and XCD k
+// (work item mod 8) takes region k, row by row, so the blocks resident on an XCD are
+// neighbours whose detector footprints overlap in its L2 (f32 72.6 against 76.1 ms with
+// consecutive tiles dealt round-robin over the XCDs, on the 512^3 / 720-view
+// backprojection; 8 x 1, 2 x 4 and 1 x 8 region grids were in between or slower).  A z
+// block has 8 equal regions, padded; padding items do nothing.  Work items of a z block
+// are numbered from a multiple of 8, so that item mod 8 is the XCD the item runs on.
+constexpr int kBpRegionsX = 4, kBpRegionsY = 2;
 static_assert(kBpRegionsX * kBpRegionsY == kXcds, "one region per XCD");
+__host__ __device__ inline int bp_region_w(int tiles_x) { return (tiles_x + kBpRegionsX - 1) / kBpRegionsX; }
+__host__ __device__ inline int bp_region_h(int tiles_y) { return (tiles_y + kBpRegionsY - 1) / kBpRegionsY; }
+// Work items per z block (the padded tile count).
+__host__ __device__ inline int bp_items_per_z(int tiles_x, int tiles_y) {
+  return kXcds * bp_region_w(tiles_x) * bp_region_h(tiles_y);
+}
+// Tile (tx, ty) of item r of a z block; false for a padding item.
+__device__ __forceinline__ bool bp_region_tile(int r, int tiles_x, int tiles_y, int &tx, int &ty) {
+  const int rw = bp_region_w(tiles_x), rh = bp_region_h(tiles_y);
+  const int xcd = r % kXcds, j = r / kXcds;
+  tx = (xcd % kBpRegionsX) * rw + j % rw;
+  ty = (xcd / kBpRegionsX) * rh + j / rw;
+  return tx < tiles_x && ty < tiles_y;
+}
 
 template <typename TS, bool HW, int ZPT, bool FIDX>
 __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int zb) {
@@ -648,17 +669,8 @@ __global__ void __launch_bounds__(256) cone_bp_kernel(BpProblem Q, size_t cells_
   Q.mult += b;
   Q.spacing += (size_t)b * 3;
   Q.origin += (size_t)b * 3;
-  // XCD-compact tiles: the (x, y) tiles of a z block form a 4 x 2 grid of regions and XCD
-  // k (block id mod 8) takes region k, row by row, so the blocks resident on an XCD are
-  // neighbours whose detector footprints overlap in its L2 (f32 72.6 against 76.1 ms with
-  // consecutive tiles dealt round-robin over the XCDs, on the 512^3 / 720-view
-  // backprojection; 8 x 1, 2 x 4 and 1 x 8 region grids were in between or slower).  The
-  // grid is padded to 8 equal regions; padding blocks exit.
-  const int rw = (Q.tiles_x + kBpRegionsX - 1) / kBpRegionsX;
-  const int rh = (Q.tiles_y + kBpRegionsY - 1) / kBpRegionsY;
-  const int xcd = (int)(blockIdx.x % kXcds), j = (int)(blockIdx.x / kXcds);
-  const int tx = (xcd % kBpRegionsX) * rw + j % rw, ty = (xcd / kBpRegionsX) * rh + j / rw;
-  if (tx >= Q.tiles_x || ty >= Q.tiles_y) return;
+  int tx, ty;
+  if (!bp_region_tile((int)blockIdx.x, Q.tiles_x, Q.tiles_y, tx, ty)) return;
   bp_block<TS, HW, ZPT, FIDX>(Q, tx, ty, zb);
 }
 
@@ -668,11 +680,9 @@ struct BpBatch {
   long long total;
 };
 
-// Persistent backprojection over uneven problems; item -> (problem, z block, tile row,
-// tile column), tile column fastest.  XCD x takes items x, x + 8, ... in claim order,
-// so at any time all XCDs work on neighbouring tiles of the same z block and share
-// that block's band of detector rows (of every view) in the Infinity Cache - the
-// order of the single-problem grid.
+// Persistent backprojection over uneven problems; item -> (problem, z block, tile of the
+// z block in region order).  XCD x takes items x, x + 8, ... in claim order, so each XCD
+// walks its own region of every z block, as in the single-problem grid.
 template <typename TS, bool HW, int ZPT, bool FIDX>
 __global__ void __launch_bounds__(256)
 cone_bp_persistent_kernel(BpBatch B, unsigned long long *counters) {
@@ -684,17 +694,17 @@ cone_bp_persistent_kernel(BpBatch B, unsigned long long *counters) {
     while (k + 1 < B.count && w >= B.p[k + 1].begin) ++k;
     const BpProblem &Q = B.p[k];
     const long long q = w - Q.begin;
-    const long long per_z = (long long)Q.tiles_x * Q.tiles_y;
+    const int per_z = bp_items_per_z(Q.tiles_x, Q.tiles_y);
     const int zb = (int)(q / per_z);
-    const long long r = q - (long long)zb * per_z;
-    const int ty = (int)(r / Q.tiles_x), tx = (int)(r - (long long)ty * Q.tiles_x);
-    bp_block<TS, HW, ZPT, FIDX>(Q, tx, ty, zb);
+    int tx, ty;
+    if (bp_region_tile((int)(q - (long long)zb * per_z), Q.tiles_x, Q.tiles_y, tx, ty))
+      bp_block<TS, HW, ZPT, FIDX>(Q, tx, ty, zb);
   }
 }
 
 // The same work list as one workgroup per item in item order (no claiming): the hardware
-// dispatches workgroups round-robin over the XCDs in block order, which is the order the
-// single-problem grid has.  The item decode is wave-uniform (scalar), so the kernel keeps
+// dispatches workgroups round-robin over the XCDs in block order, so item mod 8 is the XCD,
+// as in the single-problem grid.  The item decode is wave-uniform (scalar), so the kernel keeps
 // the single kernel's register budget; the persistent kernel holds its claim loop
 // around the block and needs more.
 template <typename TS, bool HW, int ZPT, bool FIDX>
@@ -704,10 +714,10 @@ __global__ void __launch_bounds__(256) cone_bp_items_kernel(BpBatch B) {
   while (k + 1 < B.count && w >= B.p[k + 1].begin) ++k;
   const BpProblem &Q = B.p[k];
   const int q = (int)(w - Q.begin);
-  const int per_z = Q.tiles_x * Q.tiles_y;
+  const int per_z = bp_items_per_z(Q.tiles_x, Q.tiles_y);
   const int zb = q / per_z;
-  const int r = q - zb * per_z;
-  const int ty = r / Q.tiles_x, tx = r - ty * Q.tiles_x;
+  int tx, ty;
+  if (!bp_region_tile(q - zb * per_z, Q.tiles_x, Q.tiles_y, tx, ty)) return;
   bp_block<TS, HW, ZPT, FIDX>(Q, tx, ty, zb);
 }
 
@@ -787,9 +797,7 @@ static void launch_bp(bool hw, const void *cells, float *vol, const float *P, co
   const int zblocks = (z1 - z0 + Z - 1) / Z;
   BpProblem Q{cells, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd,
               (W + 15) / 16, (H + 15) / 16, zblocks, 0, flags, z0, z1};
-  const int rw = ((W + 15) / 16 + kBpRegionsX - 1) / kBpRegionsX;
-  const int rh = ((H + 15) / 16 + kBpRegionsY - 1) / kBpRegionsY;
-  dim3 grid(kXcds * rw * rh, 1, zblocks * B);
+  dim3 grid((unsigned)bp_items_per_z((W + 15) / 16, (H + 15) / 16), 1, zblocks * B);
   const size_t cs = det_cells(n, Hd, Wd) * 4;
   if (det_fidx(Hd, Wd)) {
     if (hw)
@@ -1120,7 +1128,7 @@ static int bp_batched(const pyronn_cone_problem *pr, int count, bool hw, int bpc
     bb.p[k] = BpProblem{cells, p.output, p.projection_matrices, p.projection_multiplier,
                         p.volume_spacing, p.volume_origin, p.W, p.H, p.D, p.n_views, p.det_h,
                         p.det_w, tx, ty, zblocks, begin, kBpScale, 0, p.D};
-    begin += (long long)tx * ty * zblocks;
+    begin += (long long)bp_items_per_z(tx, ty) * zblocks;  // a multiple of 8
   }
   bb.count = count;
   bb.total = begin;
