@@ -642,6 +642,26 @@ __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int
     const float tu = fmaf(m[1], y, m[0] * x);
     const float tv = fmaf(m[5], y, m[4] * x);
     const float tw = fmaf(m[9], y, m[8] * x);
+    // Detector column axis and depth perpendicular to z (m[2] = m[10] = 0, as for any
+    // orbit about z with an untilted detector): the reference's per-voxel w and u are
+    // then the same floats for all z of the lane.  fmaf(0, z, t) + c does not depend on
+    // z unless it is a signed zero; w0 != 0 excludes that case for w, and u = +-0 samples
+    // the detector identically (the fraction u - floor(u) is +0 either way), so one
+    // reciprocal serves the lane's ZPT voxels, bit for bit.
+    if (m[2] == 0.0f && m[10] == 0.0f) {
+      const float w0 = fmaf(m[10], z[0], tw) + m[11];
+      if (w0 != 0.0f) {
+        const float ww = rcp_rn(w0);
+        const float uu = (fmaf(m[2], z[0], tu) + m[3]) * ww;
+#pragma unroll
+        for (int q = 0; q < ZPT; ++q) {
+          const float vv = (fmaf(m[6], z[q], tv) + m[7]) * ww;
+          const float f = sample_det<TS, HW, FIDX>(img, Wc, uu, vv, fW, fH);
+          acc[q] = fmaf(f * ww, ww, acc[q]);
+        }
+        continue;
+      }
+    }
 #pragma unroll
     for (int q = 0; q < ZPT; ++q) {
       const float ww = rcp_rn(fmaf(m[10], z[q], tw) + m[11]);  // correctly rounded, as the reference's division
@@ -660,7 +680,7 @@ __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int
 }
 
 template <typename TS, bool HW, int ZPT, bool FIDX>
-__global__ void __launch_bounds__(256) cone_bp_kernel(BpProblem Q, size_t cells_stride) {
+__global__ void __launch_bounds__(256, 5) cone_bp_kernel(BpProblem Q, size_t cells_stride) {
   const int b = blockIdx.z / Q.zblocks;
   const int zb = blockIdx.z - b * Q.zblocks;
   Q.cells = static_cast<const TS *>(Q.cells) + (size_t)b * cells_stride;
@@ -708,7 +728,7 @@ cone_bp_persistent_kernel(BpBatch B, unsigned long long *counters) {
 // the single kernel's register budget; the persistent kernel holds its claim loop
 // around the block and needs more.
 template <typename TS, bool HW, int ZPT, bool FIDX>
-__global__ void __launch_bounds__(256) cone_bp_items_kernel(BpBatch B) {
+__global__ void __launch_bounds__(256, 5) cone_bp_items_kernel(BpBatch B) {
   const long long w = blockIdx.x;
   int k = 0;
   while (k + 1 < B.count && w >= B.p[k + 1].begin) ++k;
