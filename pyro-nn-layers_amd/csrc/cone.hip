@@ -78,8 +78,9 @@ __global__ void cone_setup_kernel(const float *__restrict__ P, const float *__re
 // 16-byte load per sample, occupies as many bytes as the f32 quads: on the 512^3 /
 // 720-view forward it took 473 ms against 389 ms for these quads in bf16, as the 16-bit
 // gathers are limited by L2 misses rather than by instruction issue.)
-// The build goes through a z-fastest transpose of the volume so that both of its
-// passes read and write contiguously.
+// The build stages (x, z) tiles of two neighbouring y rows of the volume in LDS, so that
+// it reads the volume along x and writes the cells along z, both contiguously, with no
+// intermediate copy of the volume.
 // =====================================================================================
 template <typename TS> struct CellLayout {
   static constexpr int kPerCell = 4;  // elements per cell (an xy quad)
@@ -94,60 +95,76 @@ template <typename TS> static size_t cell_elems(int D, int H, int W) {
   return (size_t)(W + 2) * (H + 2) * (D + CellLayout<TS>::kZExtra) * CellLayout<TS>::kPerCell;
 }
 
+// The four neighbours of a cell in one store: 16 bytes (f32) or 8 bytes (16-bit types).
+__device__ __forceinline__ void store_cell(float *o, float a, float b, float c, float d) {
+  *reinterpret_cast<float4 *>(o) = make_float4(a, b, c, d);
+}
+template <typename TS>
+__device__ __forceinline__ void store_cell(TS *o, TS a, TS b, TS c, TS d) {
+  static_assert(sizeof(TS) == 2, "16-bit storage");
+  uint16_t h[4];
+  __builtin_memcpy(&h[0], &a, 2);
+  __builtin_memcpy(&h[1], &b, 2);
+  __builtin_memcpy(&h[2], &c, 2);
+  __builtin_memcpy(&h[3], &d, 2);
+  *reinterpret_cast<uint2 *>(o) =
+      make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+}
+
+// One tile = cells (cx0 .. cx0 + 63, cy, cz0 .. cz0 + 31) in cell coordinates (voxel
+// x = cx - 1, y = cy - 1, z = cz - 1).  The block reads the voxels x = cx0 - 1 .. cx0 + 63
+// of rows y and y + 1 at the tile's 32 z positions into LDS (one wave row per (y, z),
+// contiguous in x; zero outside the volume), then writes each cell's quad, lanes along
+// z (32 consecutive 16- or 8-byte cells per half wave).  Tiles are numbered with cy
+// fastest, so blocks running together read overlapping rows.
+constexpr int kCellTileX = 64, kCellTileZ = 32;
+
 template <typename TS>
 __global__ void __launch_bounds__(256)
-transpose_zx_kernel(const TS *__restrict__ vol, TS *__restrict__ vt, int W, int H, int D) {
-  // vol [b][z][y][x] -> vt [b][x][y][z]; one 32 x 32 (z, x) tile of one y plane per block
-  __shared__ TS tile[32][33];
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  const int x0 = blockIdx.x * 32, z0 = (blockIdx.y % ((D + 31) / 32)) * 32;
-  const int y = blockIdx.y / ((D + 31) / 32);
-  const int b = blockIdx.z;
-  const size_t plane = (size_t)W * H * D;
-  const TS *src = vol + b * plane;
-  TS *dst = vt + b * plane;
-#pragma unroll
-  for (int k = 0; k < 32; k += 8) {
-    const int z = z0 + ty + k, x = x0 + tx;
-    if (z < D && x < W) tile[ty + k][tx] = src[((size_t)z * H + y) * W + x];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < 32; k += 8) {
-    const int x = x0 + ty + k, z = z0 + tx;
-    if (z < D && x < W) dst[((size_t)x * H + y) * D + z] = tile[tx][ty + k];
-  }
-}
-
-template <typename TS>
-__device__ __forceinline__ TS voxt(const TS *__restrict__ vt, int x, int y, int z, int W, int H,
-                                   int D) {
-  if (x < 0 || x >= W || y < 0 || y >= H || z < 0 || z >= D) return zero_of<TS>();
-  return vt[((size_t)x * H + y) * D + z];
-}
-
-template <typename TS>
-__global__ void build_cells_kernel(const TS *__restrict__ vt, TS *__restrict__ cells, int W,
-                                   int H, int D, long long total) {
+build_cells_kernel(const TS *__restrict__ vol, TS *__restrict__ cells, int W, int H, int D,
+                   int batch) {
   using L = CellLayout<TS>;
+  __shared__ TS t[2][kCellTileZ][kCellTileX + 1];
   const int Wc = W + 2, Hc = H + 2, Dc = D + L::kZExtra;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-       t += (long long)gridDim.x * blockDim.x) {
-    long long r = t;
-    const int cz = (int)(r % Dc); r /= Dc;
+  const long long ntx = (Wc + kCellTileX - 1) / kCellTileX, ntz = (Dc + kCellTileZ - 1) / kCellTileZ;
+  const long long tiles = ntx * ntz * Hc * batch;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const size_t vstride = (size_t)W * H * D;
+  for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    long long r = tile;
     const int cy = (int)(r % Hc); r /= Hc;
-    const int cx = (int)(r % Wc);
-    const long long b = r / Wc;
-    const TS *v = vt + (size_t)b * W * H * D;
-    const int x = cx - 1, y = cy - 1, z = cz - 1;
-    TS c[4];
-    c[0] = voxt(v, x, y, z, W, H, D);
-    c[1] = voxt(v, x + 1, y, z, W, H, D);
-    c[2] = voxt(v, x, y + 1, z, W, H, D);
-    c[3] = voxt(v, x + 1, y + 1, z, W, H, D);
-    TS *o = cells + (size_t)t * L::kPerCell;
+    const int tx = (int)(r % ntx); r /= ntx;
+    const int tz = (int)(r % ntz);
+    const long long b = r / ntz;
+    const int cx0 = tx * kCellTileX, cz0 = tz * kCellTileZ;
+    const TS *v = vol + (size_t)b * vstride;
+    // 64 rows of (y, z): wave w loads rows w, w + 4, ...; lane l loads voxel x = cx0 - 1 + l,
+    // lane 0 also the 65th voxel x = cx0 + 63
+    for (int row = wave; row < 2 * kCellTileZ; row += 4) {
+      const int ry = row / kCellTileZ, rz = row % kCellTileZ;
+      const int y = cy - 1 + ry, z = cz0 - 1 + rz;
+      const bool yz_in = y >= 0 && y < H && z >= 0 && z < D;
+      const TS *vr = v + ((size_t)z * H + y) * W;
+      const int x = cx0 - 1 + lane;
+      t[ry][rz][lane] = (yz_in && x >= 0 && x < W) ? vr[x] : zero_of<TS>();
+      if (lane == 0) {
+        const int x2 = cx0 + kCellTileX - 1;
+        t[ry][rz][kCellTileX] = (yz_in && x2 < W) ? vr[x2] : zero_of<TS>();
+      }
+    }
+    __syncthreads();
+    const int j = threadIdx.x & 31;
+    const int cz = cz0 + j;
 #pragma unroll
-    for (int q = 0; q < L::kPerCell; ++q) o[q] = c[q];
+    for (int k = 0; k < kCellTileX / 8; ++k) {
+      const int i = (threadIdx.x >> 5) + 8 * k;
+      const int cx = cx0 + i;
+      if (cx < Wc && cz < Dc) {
+        TS *o = cells + ((((size_t)b * Wc + cx) * Hc + cy) * Dc + cz) * L::kPerCell;
+        store_cell(o, t[0][j][i], t[0][j][i + 1], t[1][j][i], t[1][j][i + 1]);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -165,19 +182,25 @@ __device__ __forceinline__ void unpack2(uint32_t w, _Float16, float &a, float &b
 // The eight corners of the cell a lane read last.  Consecutive samples of a ray are
 // half a voxel apart, so about half of them fall in the cell of the sample before;
 // those reuse the registers instead of loading again.  The blend is the same
-// arithmetic on the same values either way.  2^32 - 1 is never a cell index (the
-// host keeps the cell count below 2^31), so the first sample of a ray always loads.
+// arithmetic on the same values either way.  The all-ones key is never a cell index
+// (the 32-bit index serves cell counts below 2^31, the 64-bit one counts below 2^63),
+// so the first sample of a ray always loads.
 // f32 and bf16 keep the corners as floats (bf16 widened once per load).  f16 keeps the
 // packed cell: the blend's FMAs read the halves directly (v_fma_mix_f32 widens an f16
 // operand exactly inside the FMA, one rounding as before), so a load costs no
 // conversions at all.
-template <typename TS> struct CellRegs {
-  unsigned cell = 0xffffffffu;
+// IT is the index type of the cell array: int for arrays within the 32-bit limits of
+// fwd_small_index, long long beyond them.
+template <typename IT> struct CellKey { typedef unsigned type; };
+template <> struct CellKey<long long> { typedef unsigned long long type; };
+
+template <typename TS, typename IT> struct CellRegs {
+  typename CellKey<IT>::type cell = ~(typename CellKey<IT>::type)0;
   float c[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   __device__ __forceinline__ float corner(int i) const { return c[i]; }
 };
-template <> struct CellRegs<_Float16> {
-  unsigned cell = 0xffffffffu;
+template <typename IT> struct CellRegs<_Float16, IT> {
+  typename CellKey<IT>::type cell = ~(typename CellKey<IT>::type)0;
   uint32_t w[4] = {0u, 0u, 0u, 0u};  // corners 2k (low half) and 2k + 1 (high half)
   __device__ __forceinline__ float corner(int i) const {
     _Float16 h[2];
@@ -190,14 +213,17 @@ template <> struct CellRegs<_Float16> {
 // cone_projector_3D_CudaKernel.cu.cc:74-82.  With CLAMP the coordinate is clamped into
 // the zero border first (identical results for every finite coordinate); without it the
 // caller guarantees every coordinate lies in [-1, N + 1).
-// Dc = cells along z, HDc = (H + 2) * Dc: strides of the z-fastest cell array.  The
-// (y, z) part of the index, fy * Dc + fz, is an integer of magnitude below 2^23 (the
-// host keeps HDc below 2^23), so one float FMA forms it exactly; the x part uses a
-// signed 24-bit multiply (the host keeps W + 2 below 2^23 and the cell count below 2^31).
-template <typename TS, bool CLAMP>
-__device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int Dc, int HDc,
+// Dc = cells along z, HDc = (H + 2) * Dc: strides of the z-fastest cell array.
+//  * IT = int: the (y, z) part of the index, fy * Dc + fz, is an integer of magnitude
+//    below 2^23 (the host keeps HDc below 2^23), so one float FMA forms it exactly; the
+//    x part uses a signed 24-bit multiply (the host keeps W + 2 below 2^23 and the cell
+//    count below 2^31).
+//  * IT = long long: 64-bit integer products, for any volume whose sides are below 2^24
+//    (so that every coordinate and its floor are exact floats).
+template <typename TS, bool CLAMP, typename IT>
+__device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, IT Dc, IT HDc,
                                               float px, float py, float pz, float fW, float fH,
-                                              float fD, CellRegs<TS> &r) {
+                                              float fD, CellRegs<TS, IT> &r) {
   if (CLAMP) {
     px = clamp_apron(px, fW);
     py = clamp_apron(py, fH);
@@ -205,8 +231,14 @@ __device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int 
   }
   const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
   const float xd = px - fx, yd = py - fy, zd = pz - fz;
-  const int yz = (int)fmaf(fy, (float)Dc, fz);
-  const unsigned cell = (unsigned)(__mul24((int)fx, HDc) + yz + (HDc + Dc + 1));
+  typename CellKey<IT>::type cell;
+  if constexpr (sizeof(IT) == 4) {
+    const int yz = (int)fmaf(fy, (float)Dc, fz);
+    cell = (unsigned)(__mul24((int)fx, HDc) + yz + (HDc + Dc + 1));
+  } else {
+    cell = (unsigned long long)((long long)(int)fx * HDc + (long long)(int)fy * Dc + (long long)(int)fz +
+                                (HDc + Dc + 1));
+  }
   if (cell != r.cell) {
     r.cell = cell;
     if constexpr (sizeof(TS) == 4) {
@@ -237,47 +269,47 @@ __device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, int 
   return lerp_ref(p0, p1, xd);
 }
 
-template <typename TS, bool HW, bool CLAMP>
-__device__ __forceinline__ float cone_sample(const TS *__restrict__ c, int Dc, int HDc, float px,
+template <typename TS, bool HW, bool CLAMP, typename IT>
+__device__ __forceinline__ float cone_sample(const TS *__restrict__ c, IT Dc, IT HDc, float px,
                                              float py, float pz, float fW, float fH, float fD,
-                                             CellRegs<TS> &r) {
+                                             CellRegs<TS, IT> &r) {
   if (HW) {  // tex3D(p + 0.5): the sampler subtracts the half texel again
     px = (px + 0.5f) - 0.5f;
     py = (py + 0.5f) - 0.5f;
     pz = (pz + 0.5f) - 0.5f;
   }
-  return sample_cells<TS, CLAMP>(c, Dc, HDc, px, py, pz, fW, fH, fD, r);
+  return sample_cells<TS, CLAMP, IT>(c, Dc, HDc, px, py, pz, fW, fH, fD, r);
 }
 
 // The march of one ray from its clipped parameter interval: entrance sample with weight
 // 1/2, loop samples, and the tail rule (cone_projector_3D_CudaKernel.cu.cc:134-179).
 // GUARDED = false drops the coordinate clamp and the iteration cap; the caller uses it
 // only for rays where neither can change anything (see fwd_block).
-template <typename TS, bool HW, bool GUARDED>
-__device__ __forceinline__ float march_ray(const TS *__restrict__ v, int Dc, int HDc, float fW,
+template <typename TS, bool HW, bool GUARDED, typename IT>
+__device__ __forceinline__ float march_ray(const TS *__restrict__ v, IT Dc, IT HDc, float fW,
                                            float fH, float fD, float rx, float ry, float rz,
                                            float s0, float s1, float s2, float amin, float amax,
                                            float step) {
   float pix = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
-  CellRegs<TS> regs;
+  CellRegs<TS, IT> regs;
   const int cap = GUARDED ? march_cap(amin, amax, step) : 0;
   if (amin < amax) {
     px = fmaf(amin, rx, s0);
     py = fmaf(amin, ry, s1);
     pz = fmaf(amin, rz, s2);
-    pix = fmaf(0.5f, cone_sample<TS, HW, GUARDED>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs), pix);
+    pix = fmaf(0.5f, cone_sample<TS, HW, GUARDED, IT>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs), pix);
     amin += step;
   }
   for (int it = 0; amin < amax && (!GUARDED || it < cap); ++it) {
     px = fmaf(amin, rx, s0);
     py = fmaf(amin, ry, s1);
     pz = fmaf(amin, rz, s2);
-    pix += cone_sample<TS, HW, GUARDED>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs);
+    pix += cone_sample<TS, HW, GUARDED, IT>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs);
     amin += step;
   }
   pix *= step;
   if (pix > 0.0f) {  // tail only for a positive sum (cone_projector_3D_CudaKernel.cu.cc:164-179)
-    const float flast = cone_sample<TS, HW, GUARDED>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs);
+    const float flast = cone_sample<TS, HW, GUARDED, IT>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs);
     pix = fmaf(-(0.5f * step), flast, pix);
     amin -= step;
     const float last = amax - amin;
@@ -285,7 +317,7 @@ __device__ __forceinline__ float march_ray(const TS *__restrict__ v, int Dc, int
     px = fmaf(amax, rx, s0);
     py = fmaf(amax, ry, s1);
     pz = fmaf(amax, rz, s2);
-    pix = fmaf(0.5f * last, cone_sample<TS, HW, GUARDED>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs),
+    pix = fmaf(0.5f * last, cone_sample<TS, HW, GUARDED, IT>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs),
                pix);
   }
   return pix;
@@ -323,7 +355,7 @@ struct FwdProblem {
   long long begin;       // first work item of the problem in a batched launch (multiple of 8)
 };
 
-template <typename TS, bool HW>
+template <typename TS, bool HW, typename IT>
 __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int xcd = (int)(q % kXcds);
@@ -345,7 +377,7 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   const float step = P.step[0];
   const float spx = P.spacing[2], spy = P.spacing[1], spz = P.spacing[0];
   const TS *v = static_cast<const TS *>(P.cells);
-  const int Wp = D + CellLayout<TS>::kZExtra, HWp = (H + 2) * Wp;  // z-fastest cell strides
+  const IT Wp = (IT)D + CellLayout<TS>::kZExtra, HWp = (IT)(H + 2) * Wp;  // z-fastest cell strides
   const float fW = (float)W, fH = (float)H, fD = (float)D;
   float *out = P.sino + ((size_t)view * Hd + row) * (size_t)Wd + col;
   const float u = (float)col, w = (float)row;
@@ -402,24 +434,27 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
                      fmaxf(fabsf(s0), fmaxf(fabsf(s1), fabsf(s2))) < 0x1p18f && amag < 0x1p19f &&
                      amag * 0x1p-20f <= step;
   const float pix =
-      __all(plain) ? march_ray<TS, HW, false>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step)
-                   : march_ray<TS, HW, true>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step);
+      __all(plain) ? march_ray<TS, HW, false, IT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step)
+                   : march_ray<TS, HW, true, IT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step);
   const float a = rx * spx, bq = ry * spy, c = rz * spz;
   *out = pix * sqrtf(fmaf(c, c, fmaf(bq, bq, a * a)));
 }
 
-template <typename TS, bool HW>
+// One launch covers the views P.n of a range of each item's n_all views (the host splits
+// the views so that a launch stays within the dispatch grid's 2^32 work-items); P.sino,
+// P.inv_ar and P.src point at the range's first view of item 0.
+template <typename TS, bool HW, typename IT>
 __global__ void __launch_bounds__(256)
-cone_fwd_kernel(FwdProblem P, long long cell_stride) {
+cone_fwd_kernel(FwdProblem P, long long cell_stride, int n_all) {
   // blockIdx.z selects the batch item; items share shapes and have their own geometry
   const int b = blockIdx.z;
   P.cells = static_cast<const TS *>(P.cells) + (size_t)b * cell_stride;
-  P.sino += (size_t)b * P.n * P.Hd * P.Wd;
-  P.inv_ar += (size_t)b * P.n * 9;
-  P.src += (size_t)b * P.n * 3;
+  P.sino += (size_t)b * n_all * P.Hd * P.Wd;
+  P.inv_ar += (size_t)b * n_all * 9;
+  P.src += (size_t)b * n_all * 3;
   P.step += b;
   P.spacing += (size_t)b * 3;
-  fwd_block<TS, HW>(P, blockIdx.x);
+  fwd_block<TS, HW, IT>(P, blockIdx.x);
 }
 
 // Up to this many problems of different shapes run in one persistent launch.
@@ -453,7 +488,7 @@ __device__ __forceinline__ long long claim(unsigned long long *counters, int xcd
 // the single-launch order of each problem (problems concatenated, each a multiple of
 // 8 items); XCD x processes items x, x + 8, x + 16, ... in claim order, so the blocks
 // resident on an XCD always work on consecutive items of its detector row bands.
-template <typename TS, bool HW>
+template <typename TS, bool HW, typename IT>
 __global__ void __launch_bounds__(256)
 cone_fwd_persistent_kernel(FwdBatch B, unsigned long long *counters) {
   // one claim takes kFwdClaim consecutive items of the XCD's sequence (a 16 x 16 tile of
@@ -469,7 +504,7 @@ cone_fwd_persistent_kernel(FwdBatch B, unsigned long long *counters) {
       const long long w = (t0 + j) * kXcds + xcd;
       if (w >= B.total) break;
       while (k + 1 < B.count && w >= B.p[k + 1].begin) ++k;
-      fwd_block<TS, HW>(B.p[k], w - B.p[k].begin);
+      fwd_block<TS, HW, IT>(B.p[k], w - B.p[k].begin);
     }
   }
 }
@@ -485,22 +520,6 @@ template <typename TS>
 __device__ __forceinline__ TS pix(const TS *__restrict__ s, int u, int v, int Wd, int Hd) {
   if (u < 0 || u >= Wd || v < 0 || v >= Hd) return zero_of<TS>();
   return s[(size_t)v * Wd + u];
-}
-
-// The four neighbours of a cell in one store: 16 bytes (f32) or 8 bytes (16-bit types).
-__device__ __forceinline__ void store_cell(float *o, float a, float b, float c, float d) {
-  *reinterpret_cast<float4 *>(o) = make_float4(a, b, c, d);
-}
-template <typename TS>
-__device__ __forceinline__ void store_cell(TS *o, TS a, TS b, TS c, TS d) {
-  static_assert(sizeof(TS) == 2, "16-bit storage");
-  uint16_t h[4];
-  __builtin_memcpy(&h[0], &a, 2);
-  __builtin_memcpy(&h[1], &b, 2);
-  __builtin_memcpy(&h[2], &c, 2);
-  __builtin_memcpy(&h[3], &d, 2);
-  *reinterpret_cast<uint2 *>(o) =
-      make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
 }
 
 // Lanes along a row of cells (x), blocks over runs of kDetCellRows cell rows (y) and
@@ -679,8 +698,10 @@ __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int
   }
 }
 
+// Block (x, z) of a launch: item item0 + x of z block z mod zblocks of batch item
+// z / zblocks (item0 is a multiple of 8, so that block x runs on the XCD of its item).
 template <typename TS, bool HW, int ZPT, bool FIDX>
-__global__ void __launch_bounds__(256, 5) cone_bp_kernel(BpProblem Q, size_t cells_stride) {
+__global__ void __launch_bounds__(256, 5) cone_bp_kernel(BpProblem Q, size_t cells_stride, int item0) {
   const int b = blockIdx.z / Q.zblocks;
   const int zb = blockIdx.z - b * Q.zblocks;
   Q.cells = static_cast<const TS *>(Q.cells) + (size_t)b * cells_stride;
@@ -690,7 +711,7 @@ __global__ void __launch_bounds__(256, 5) cone_bp_kernel(BpProblem Q, size_t cel
   Q.spacing += (size_t)b * 3;
   Q.origin += (size_t)b * 3;
   int tx, ty;
-  if (!bp_region_tile((int)blockIdx.x, Q.tiles_x, Q.tiles_y, tx, ty)) return;
+  if (!bp_region_tile(item0 + (int)blockIdx.x, Q.tiles_x, Q.tiles_y, tx, ty)) return;
   bp_block<TS, HW, ZPT, FIDX>(Q, tx, ty, zb);
 }
 
@@ -748,48 +769,75 @@ static size_t cell_elems_dt(int dtype, int D, int H, int W) {
   return dtype == PYRONN_F32 ? cell_elems<float>(D, H, W) : cell_elems<bf16s>(D, H, W);
 }
 
-// Limits of the forward's index arithmetic: fewer than 2^31 cells, and W + 2 and the
-// (y, z) plane of cells below 2^23 for the 24-bit multiplies of the sampler.
-static bool fwd_volume_fits(int dtype, int D, int H, int W) {
-  const int zx = dtype == PYRONN_F32 ? CellLayout<float>::kZExtra : CellLayout<bf16s>::kZExtra;
-  return cell_elems_dt(dtype, D, H, W) / 4 < (size_t)INT32_MAX && W + 2 < (1 << 23) &&
-         (long long)(H + 2) * (D + zx) < (1LL << 23);
+// Volumes the forward accepts: every side below 2^24 - 3, so that each cell coordinate and
+// its floor are exact floats (the reference indexes with uint32 and so takes volumes below
+// 2^32 voxels; all of those with sides below 2^24 - 3 are accepted here, and larger ones).
+static bool fwd_volume_ok(int D, int H, int W) {
+  const int lim = (1 << 24) - 3;
+  return D < lim && H < lim && W < lim;
+}
+// Whether the forward may use the 32-bit cell index (sample_cells, IT = int): fewer than 2^31
+// cells, and W + 2 and the (y, z) plane of cells below 2^23 for the 24-bit multiply and the
+// exact float FMA of the sampler.  Larger volumes take the 64-bit index.
+static bool fwd_small_index(int D, int H, int W) {
+  const long long cells = (long long)(W + 2) * (H + 2) * (D + CellLayout<float>::kZExtra);
+  return cells < (long long)INT32_MAX && W + 2 < (1 << 23) &&
+         (long long)(H + 2) * (D + CellLayout<float>::kZExtra) < (1LL << 23);
 }
 static size_t det_cells(int n, int Hd, int Wd) { return (size_t)n * (Hd + 2) * (Wd + 2); }
 // Whether the backprojection may form detector cell offsets with one float FMA (exact below 2^24).
 static bool det_fidx(int Hd, int Wd) { return (long long)(Hd + 2) * (Wd + 2) <= (1LL << 24); }
+
+// A dispatch's grid size is a 32-bit count of work-items, so a 1-D grid of 256-lane blocks
+// holds at most 2^24 - 1 blocks; grid y and z hold at most 65535 blocks here.
+constexpr long long kMaxGridX = (1LL << 24) - 1;
+constexpr int kMaxGridYZ = 65535;
 
 // Blocks of one forward launch per batch item: 8 XCD lanes x tile-row bands x views x tiles.
 static long long fwd_blocks(int n, int Hd, int Wd) {
   const long long tiles_x = (Wd + 15) / 16, tiles_y = (Hd + 15) / 16;
   return (long long)kXcds * ((tiles_y + kXcds - 1) / kXcds) * n * tiles_x;
 }
+// Views per forward launch (0: one view alone exceeds a launch).
+static int fwd_views_per_launch(int n, int Hd, int Wd) {
+  const long long per = kMaxGridX / fwd_blocks(1, Hd, Wd);
+  return (int)(per < n ? per : n);
+}
 
 template <typename TS>
-static void launch_cells(const void *vol, void *vt, void *cells, int B, int D, int H, int W,
-                         hipStream_t stream) {
-  dim3 tgrid((W + 31) / 32, ((D + 31) / 32) * H, B);
-  transpose_zx_kernel<TS><<<tgrid, dim3(256), 0, stream>>>((const TS *)vol, (TS *)vt, W, H, D);
-  const long long total =
-      (long long)B * (long long)(cell_elems<TS>(D, H, W) / CellLayout<TS>::kPerCell);
-  long long blocks = (total + 255) / 256;
-  if (blocks > 65536 * 4) blocks = 65536 * 4;
-  build_cells_kernel<TS><<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(
-      (const TS *)vt, (TS *)cells, W, H, D, total);
+static void launch_cells(const void *vol, void *cells, int B, int D, int H, int W, hipStream_t stream) {
+  const long long tiles = (long long)((W + 2 + kCellTileX - 1) / kCellTileX) *
+                          ((D + CellLayout<TS>::kZExtra + kCellTileZ - 1) / kCellTileZ) * (H + 2) * B;
+  const long long blocks = tiles < 65536 * 8 ? tiles : 65536 * 8;
+  build_cells_kernel<TS><<<dim3((unsigned)blocks), dim3(256), 0, stream>>>((const TS *)vol, (TS *)cells, W, H,
+                                                                           D, B);
+}
+
+template <typename TS, typename IT>
+static void launch_fwd_it(bool hw, const FwdProblem &P, dim3 grid, long long cs, int n_all, hipStream_t stream) {
+  if (hw)
+    cone_fwd_kernel<TS, true, IT><<<grid, dim3(256), 0, stream>>>(P, cs, n_all);
+  else
+    cone_fwd_kernel<TS, false, IT><<<grid, dim3(256), 0, stream>>>(P, cs, n_all);
 }
 
 template <typename TS>
 static void launch_fwd(bool hw, const void *cells, float *sino, const float *inv_ar,
                        const float *src, const float *step, const float *spacing, int B, int D,
                        int H, int W, int n, int Hd, int Wd, hipStream_t stream) {
-  FwdProblem P{cells, sino, inv_ar, src, step, spacing, W, H, D, n, Hd, Wd,
-               (Wd + 15) / 16, (Hd + 15) / 16, 0};
-  dim3 grid((unsigned)fwd_blocks(n, Hd, Wd), 1, B);
   const long long cs = (long long)cell_elems<TS>(D, H, W);
-  if (hw)
-    cone_fwd_kernel<TS, true><<<grid, dim3(256), 0, stream>>>(P, cs);
-  else
-    cone_fwd_kernel<TS, false><<<grid, dim3(256), 0, stream>>>(P, cs);
+  const bool small = fwd_small_index(D, H, W);
+  const int per = fwd_views_per_launch(n, Hd, Wd);
+  for (int v0 = 0; v0 < n; v0 += per) {
+    const int k = n - v0 < per ? n - v0 : per;
+    FwdProblem P{cells, sino + (size_t)v0 * Hd * Wd, inv_ar + (size_t)v0 * 9, src + (size_t)v0 * 3, step,
+                 spacing, W, H, D, k, Hd, Wd, (Wd + 15) / 16, (Hd + 15) / 16, 0};
+    dim3 grid((unsigned)fwd_blocks(k, Hd, Wd), 1, B);
+    if (small)
+      launch_fwd_it<TS, int>(hw, P, grid, cs, n, stream);
+    else
+      launch_fwd_it<TS, long long>(hw, P, grid, cs, n, stream);
+  }
 }
 
 // z voxels per lane: 8 for f32 detector cells, 4 for the 16-bit ones (measured
@@ -801,10 +849,16 @@ static void launch_det_cells(const void *sino, void *cells, int images, int Hd, 
                              hipStream_t stream) {
   const int Hc = Hd + 2;
   const int runs = (Hc + kDetCellRows - 1) / kDetCellRows;
-  const dim3 grid((unsigned)((Wd + 2 + 255) / 256), (unsigned)(runs < 65535 ? runs : 65535),
-                  (unsigned)(images < 65535 ? images : 65535));
+  const dim3 grid((unsigned)((Wd + 2 + 255) / 256), (unsigned)(runs < kMaxGridYZ ? runs : kMaxGridYZ),
+                  (unsigned)(images < kMaxGridYZ ? images : kMaxGridYZ));
   build_det_cells_kernel<TS><<<grid, dim3(256), 0, stream>>>((const TS *)sino, (TS *)cells, Hd, Wd,
                                                              images);
+}
+
+// Whether the item numbers of one backprojection z block of a W x H volume fit an int
+// (a launch takes at most kMaxGridX of them; larger planes take several launches).
+static bool bp_plane_fits(int H, int W) {
+  return (long long)kXcds * bp_region_w((W + 15) / 16) * bp_region_h((H + 15) / 16) < (long long)INT32_MAX;
 }
 
 template <typename TS>
@@ -815,20 +869,30 @@ static void launch_bp(bool hw, const void *cells, float *vol, const float *P, co
   constexpr int Z = bp_zpt<TS>();
   if (z1 < 0) z1 = D;
   const int zblocks = (z1 - z0 + Z - 1) / Z;
-  BpProblem Q{cells, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd,
-              (W + 15) / 16, (H + 15) / 16, zblocks, 0, flags, z0, z1};
-  dim3 grid((unsigned)bp_items_per_z((W + 15) / 16, (H + 15) / 16), 1, zblocks * B);
+  // grid z = batch items x z blocks, at most 65535 per launch: split the z blocks; grid x =
+  // the items of a z block, at most kMaxGridX (rounded down to a multiple of 8) per launch
+  const int per = kMaxGridYZ / B;
   const size_t cs = det_cells(n, Hd, Wd) * 4;
-  if (det_fidx(Hd, Wd)) {
-    if (hw)
-      cone_bp_kernel<TS, true, Z, true><<<grid, dim3(256), 0, stream>>>(Q, cs);
-    else
-      cone_bp_kernel<TS, false, Z, true><<<grid, dim3(256), 0, stream>>>(Q, cs);
-  } else {
-    if (hw)
-      cone_bp_kernel<TS, true, Z, false><<<grid, dim3(256), 0, stream>>>(Q, cs);
-    else
-      cone_bp_kernel<TS, false, Z, false><<<grid, dim3(256), 0, stream>>>(Q, cs);
+  const int items = bp_items_per_z((W + 15) / 16, (H + 15) / 16);
+  const int max_x = (int)(kMaxGridX / kXcds * kXcds);
+  for (int zb0 = 0; zb0 < zblocks; zb0 += per) {
+    const int nz = zblocks - zb0 < per ? zblocks - zb0 : per;
+    BpProblem Q{cells, vol, P, mult, spacing, origin, W, H, D, n, Hd, Wd,
+                (W + 15) / 16, (H + 15) / 16, nz, 0, flags, z0 + zb0 * Z, z1};
+    for (int i0 = 0; i0 < items; i0 += max_x) {
+      dim3 grid((unsigned)(items - i0 < max_x ? items - i0 : max_x), 1, nz * B);
+      if (det_fidx(Hd, Wd)) {
+        if (hw)
+          cone_bp_kernel<TS, true, Z, true><<<grid, dim3(256), 0, stream>>>(Q, cs, i0);
+        else
+          cone_bp_kernel<TS, false, Z, true><<<grid, dim3(256), 0, stream>>>(Q, cs, i0);
+      } else {
+        if (hw)
+          cone_bp_kernel<TS, true, Z, false><<<grid, dim3(256), 0, stream>>>(Q, cs, i0);
+        else
+          cone_bp_kernel<TS, false, Z, false><<<grid, dim3(256), 0, stream>>>(Q, cs, i0);
+      }
+    }
   }
 }
 
@@ -858,8 +922,7 @@ extern "C" size_t pyronn_cone_projection3d_workspace_size(int batch, int D, int 
                                                           int n_views, int volume_dtype) {
   if (batch <= 0 || D <= 0 || H <= 0 || W <= 0 || n_views <= 0 || !dtype_ok(volume_dtype)) return 0;
   return fwd_geom_bytes(batch, n_views) +
-         align_up((size_t)batch * cell_elems_dt(volume_dtype, D, H, W) * dtype_bytes(volume_dtype), 256) +
-         align_up((size_t)batch * D * H * W * dtype_bytes(volume_dtype), 256);
+         align_up((size_t)batch * cell_elems_dt(volume_dtype, D, H, W) * dtype_bytes(volume_dtype), 256);
 }
 
 extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, int batch, int D,
@@ -876,20 +939,18 @@ extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, in
                        sinogram && workspace,
                    "cone_projection3d: null pointer");
   PYRONN_CHECK_ARG(dtype_ok(volume_dtype), "cone_projection3d: bad volume dtype %d", volume_dtype);
-  PYRONN_CHECK_ARG(batch > 0 && batch <= 65535 && D > 0 && H > 0 && W > 0 && n_views > 0 &&
-                       n_views <= 65535 && det_h > 0 && det_w > 0,
+  PYRONN_CHECK_ARG(batch > 0 && batch <= kMaxGridYZ && D > 0 && H > 0 && W > 0 && n_views > 0 &&
+                       det_h > 0 && det_w > 0,
                    "cone_projection3d: bad shape B=%d D=%d H=%d W=%d n=%d det=%dx%d", batch, D, H, W,
                    n_views, det_h, det_w);
-  if (!fwd_volume_fits(volume_dtype, D, H, W)) {
-    set_error("cone_projection3d: volume %dx%dx%d exceeds the cell index limits", D, H, W);
+  PYRONN_CHECK_ARG((long long)batch * n_views < (long long)INT32_MAX,
+                   "cone_projection3d: batch x views %lld too large", (long long)batch * n_views);
+  if (!fwd_volume_ok(D, H, W)) {
+    set_error("cone_projection3d: volume %dx%dx%d: a side of 2^24 - 3 voxels or more", D, H, W);
     return PYRONN_ERR_TOO_LARGE;
   }
-  if ((long long)((D + 31) / 32) * H > 65535) {
-    set_error("cone_projection3d: volume %dx%dx%d too large for the transpose grid", D, H, W);
-    return PYRONN_ERR_TOO_LARGE;
-  }
-  if (fwd_blocks(n_views, det_h, det_w) >= (long long)INT32_MAX) {
-    set_error("cone_projection3d: %d views of %dx%d pixels exceed one launch", n_views, det_h, det_w);
+  if (fwd_views_per_launch(n_views, det_h, det_w) < 1) {
+    set_error("cone_projection3d: detector %dx%d exceeds one launch", det_h, det_w);
     return PYRONN_ERR_TOO_LARGE;
   }
   const size_t need = pyronn_cone_projection3d_workspace_size(batch, D, H, W, n_views, volume_dtype);
@@ -903,27 +964,46 @@ extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, in
   float *inv_ar = (float *)ws;
   float *srcp = inv_ar + (size_t)batch * n_views * 9;
   void *cells = ws + fwd_geom_bytes(batch, n_views);
-  void *vt = (char *)cells +
-             align_up((size_t)batch * cell_elems_dt(volume_dtype, D, H, W) * dtype_bytes(volume_dtype), 256);
 
   int rc = pyronn_cone_geometry_setup(projection_matrices, volume_origin, volume_spacing, batch,
                                       n_views, inv_ar, srcp, stream);
   if (rc != PYRONN_OK) return rc;
   const bool hw = hardware_interp != 0;
   if (volume_dtype == PYRONN_F32) {
-    launch_cells<float>(volume, vt, cells, batch, D, H, W, s);
+    launch_cells<float>(volume, cells, batch, D, H, W, s);
     launch_fwd<float>(hw, cells, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H, W,
                       n_views, det_h, det_w, s);
   } else if (volume_dtype == PYRONN_BF16) {
-    launch_cells<bf16s>(volume, vt, cells, batch, D, H, W, s);
+    launch_cells<bf16s>(volume, cells, batch, D, H, W, s);
     launch_fwd<bf16s>(hw, cells, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H, W,
                       n_views, det_h, det_w, s);
   } else {
-    launch_cells<_Float16>(volume, vt, cells, batch, D, H, W, s);
+    launch_cells<_Float16>(volume, cells, batch, D, H, W, s);
     launch_fwd<_Float16>(hw, cells, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H,
                          W, n_views, det_h, det_w, s);
   }
   PYRONN_CHECK_LAUNCH("cone_fwd_kernel");
+  return PYRONN_OK;
+}
+
+// Shape checks shared by the backprojection entry points: the detector cell offset of the
+// sampler (24-bit multiplies, below 2^31 per image), one z block of the volume per launch
+// row, and batch x views images indexed with int.
+static int bp_check_shape(const char *name, int batch, int D, int H, int W, int n, int det_h, int det_w) {
+  PYRONN_CHECK_ARG(batch > 0 && batch <= kMaxGridYZ && D > 0 && H > 0 && W > 0 && n > 0 && det_h > 0 &&
+                       det_w > 0,
+                   "%s: bad shape B=%d D=%d H=%d W=%d n=%d det=%dx%d", name, batch, D, H, W, n, det_h, det_w);
+  PYRONN_CHECK_ARG((long long)batch * n < (long long)INT32_MAX, "%s: batch x views %lld too large", name,
+                   (long long)batch * n);
+  if ((size_t)(det_h + 2) * (det_w + 2) >= (size_t)INT32_MAX / 4 || det_w + 2 >= (1 << 23) ||
+      det_h + 2 >= (1 << 23)) {
+    set_error("%s: detector %dx%d too large", name, det_h, det_w);
+    return PYRONN_ERR_TOO_LARGE;
+  }
+  if (!bp_plane_fits(H, W)) {
+    set_error("%s: volume plane %dx%d too large for one launch", name, H, W);
+    return PYRONN_ERR_TOO_LARGE;
+  }
   return PYRONN_OK;
 }
 
@@ -951,16 +1031,8 @@ extern "C" int pyronn_cone_backprojection3d(const void *sinogram, int sinogram_d
                    "cone_backprojection3d: null pointer");
   PYRONN_CHECK_ARG(dtype_ok(sinogram_dtype), "cone_backprojection3d: bad sinogram dtype %d",
                    sinogram_dtype);
-  PYRONN_CHECK_ARG(batch > 0 && D > 0 && H > 0 && W > 0 && n_views > 0 && det_h > 0 && det_w > 0,
-                   "cone_backprojection3d: bad shape B=%d D=%d H=%d W=%d n=%d det=%dx%d", batch, D,
-                   H, W, n_views, det_h, det_w);
-  PYRONN_CHECK_ARG((long long)((D + 3) / 4) * batch <= 65535,
-                   "cone_backprojection3d: batch x depth too large for one launch");
-  if ((size_t)(det_h + 2) * (det_w + 2) >= (size_t)INT32_MAX / 4 || det_w + 2 >= (1 << 23) ||
-      det_h + 2 >= (1 << 23)) {
-    set_error("cone_backprojection3d: detector too large");
-    return PYRONN_ERR_TOO_LARGE;
-  }
+  const int rc = bp_check_shape("cone_backprojection3d", batch, D, H, W, n_views, det_h, det_w);
+  if (rc != PYRONN_OK) return rc;
   const size_t need =
       pyronn_cone_backprojection3d_workspace_size(batch, n_views, det_h, det_w, sinogram_dtype);
   if (workspace_bytes < need) {
@@ -1001,18 +1073,10 @@ extern "C" int pyronn_cone_backprojection3d_views(
   PYRONN_CHECK_ARG(dtype_ok(sinogram_dtype), "cone_backprojection3d_views: bad sinogram dtype %d",
                    sinogram_dtype);
   PYRONN_CHECK_ARG(mode >= 0 && mode <= 7, "cone_backprojection3d_views: bad mode %d", mode);
-  PYRONN_CHECK_ARG(batch > 0 && D > 0 && H > 0 && W > 0 && n_views > 0 && det_h > 0 && det_w > 0,
-                   "cone_backprojection3d_views: bad shape B=%d D=%d H=%d W=%d n=%d det=%dx%d", batch,
-                   D, H, W, n_views, det_h, det_w);
+  const int rc = bp_check_shape("cone_backprojection3d_views", batch, D, H, W, n_views, det_h, det_w);
+  if (rc != PYRONN_OK) return rc;
   PYRONN_CHECK_ARG(z_begin >= 0 && z_begin < z_end && z_end <= D,
                    "cone_backprojection3d_views: bad z range [%d, %d) for depth %d", z_begin, z_end, D);
-  PYRONN_CHECK_ARG((long long)((z_end - z_begin + 3) / 4) * batch <= 65535,
-                   "cone_backprojection3d_views: batch x depth too large for one launch");
-  if ((size_t)(det_h + 2) * (det_w + 2) >= (size_t)INT32_MAX / 4 || det_w + 2 >= (1 << 23) ||
-      det_h + 2 >= (1 << 23)) {
-    set_error("cone_backprojection3d_views: detector too large");
-    return PYRONN_ERR_TOO_LARGE;
-  }
   const size_t need =
       pyronn_cone_backprojection3d_workspace_size(batch, n_views, det_h, det_w, sinogram_dtype);
   if (workspace_bytes < need) {
@@ -1073,16 +1137,14 @@ static int check_problems(const char *name, const pyronn_cone_problem *pr, int c
     PYRONN_CHECK_ARG(p.D > 0 && p.H > 0 && p.W > 0 && p.n_views > 0 && p.det_h > 0 && p.det_w > 0,
                      "%s: bad shape in problem %d", name, k);
     if (forward) {
-      if (!fwd_volume_fits(dtype, p.D, p.H, p.W) ||
-          (long long)((p.D + 31) / 32) * p.H > 65535 ||
-          fwd_blocks(p.n_views, p.det_h, p.det_w) >= (long long)INT32_MAX) {
-        set_error("%s: problem %d too large", name, k);
+      if (!fwd_volume_ok(p.D, p.H, p.W)) {
+        set_error("%s: problem %d: volume %dx%dx%d has a side of 2^24 - 3 voxels or more", name, k, p.D, p.H,
+                  p.W);
         return PYRONN_ERR_TOO_LARGE;
       }
-    } else if ((size_t)(p.det_h + 2) * (p.det_w + 2) >= (size_t)INT32_MAX / 4 ||
-               p.det_w + 2 >= (1 << 23) || p.det_h + 2 >= (1 << 23)) {
-      set_error("%s: problem %d detector too large", name, k);
-      return PYRONN_ERR_TOO_LARGE;
+    } else {
+      const int rc = bp_check_shape(name, 1, p.D, p.H, p.W, p.n_views, p.det_h, p.det_w);
+      if (rc != PYRONN_OK) return rc;
     }
   }
   return PYRONN_OK;
@@ -1090,8 +1152,7 @@ static int check_problems(const char *name, const pyronn_cone_problem *pr, int c
 
 static size_t fwd_problem_bytes(const pyronn_cone_problem &p, int dtype) {
   const size_t e = dtype_bytes(dtype);
-  return fwd_geom_bytes(1, p.n_views) + align_up(cell_elems_dt(dtype, p.D, p.H, p.W) * e, 256) +
-         align_up((size_t)p.D * p.H * p.W * e, 256);
+  return fwd_geom_bytes(1, p.n_views) + align_up(cell_elems_dt(dtype, p.D, p.H, p.W) * e, 256);
 }
 
 constexpr size_t kCounterBytes = 256;  // per-XCD work counters at the start of a batched workspace
@@ -1103,16 +1164,17 @@ static int fwd_batched(const pyronn_cone_problem *pr, int count, bool hw, int bp
   ws += kCounterBytes;
   FwdBatch fb{};
   long long begin = 0;
+  bool small = true;
   for (int k = 0; k < count; ++k) {
     const pyronn_cone_problem &p = pr[k];
     float *inv_ar = (float *)ws;
     float *src = inv_ar + (size_t)p.n_views * 9;
     char *cells = ws + fwd_geom_bytes(1, p.n_views);
-    char *vt = cells + align_up(cell_elems<TS>(p.D, p.H, p.W) * sizeof(TS), 256);
     ws += fwd_problem_bytes(p, sizeof(TS) == 4 ? PYRONN_F32 : PYRONN_BF16);
     cone_setup_kernel<<<dim3((p.n_views + 127) / 128), dim3(128), 0, s>>>(
         p.projection_matrices, p.volume_origin, p.volume_spacing, 1, p.n_views, inv_ar, src);
-    launch_cells<TS>(p.input, vt, cells, 1, p.D, p.H, p.W, s);
+    launch_cells<TS>(p.input, cells, 1, p.D, p.H, p.W, s);
+    small = small && fwd_small_index(p.D, p.H, p.W);
     fb.p[k] = FwdProblem{cells, p.output, inv_ar, src, p.step_size, p.volume_spacing,
                          p.W, p.H, p.D, p.n_views, p.det_h, p.det_w,
                          (p.det_w + 15) / 16, (p.det_h + 15) / 16, begin};
@@ -1124,10 +1186,14 @@ static int fwd_batched(const pyronn_cone_problem *pr, int count, bool hw, int bp
   grid = (grid + kXcds - 1) / kXcds * kXcds;
   if (grid > begin) grid = begin;
   reset_counters_kernel<<<1, 64, 0, s>>>(counters);
-  if (hw)
-    cone_fwd_persistent_kernel<TS, true><<<dim3((unsigned)grid), dim3(256), 0, s>>>(fb, counters);
-  else
-    cone_fwd_persistent_kernel<TS, false><<<dim3((unsigned)grid), dim3(256), 0, s>>>(fb, counters);
+  const dim3 g((unsigned)grid);
+  if (small) {
+    if (hw) cone_fwd_persistent_kernel<TS, true, int><<<g, dim3(256), 0, s>>>(fb, counters);
+    else cone_fwd_persistent_kernel<TS, false, int><<<g, dim3(256), 0, s>>>(fb, counters);
+  } else {
+    if (hw) cone_fwd_persistent_kernel<TS, true, long long><<<g, dim3(256), 0, s>>>(fb, counters);
+    else cone_fwd_persistent_kernel<TS, false, long long><<<g, dim3(256), 0, s>>>(fb, counters);
+  }
   return PYRONN_OK;
 }
 
@@ -1154,7 +1220,7 @@ static int bp_batched(const pyronn_cone_problem *pr, int count, bool hw, int bpc
   bb.total = begin;
   bool fidx = true;
   for (int k = 0; k < count; ++k) fidx = fidx && det_fidx(pr[k].det_h, pr[k].det_w);
-  if (bpc <= 0 && begin < (long long)INT32_MAX) {  // one workgroup per item (the default)
+  if (bpc <= 0 && begin <= kMaxGridX) {  // one workgroup per item (the default; else the persistent grid)
     const dim3 g((unsigned)begin);
     if (fidx) {
       if (hw) cone_bp_items_kernel<TS, true, Z, true><<<g, dim3(256), 0, s>>>(bb);
@@ -1257,3 +1323,4 @@ extern "C" int pyronn_cone_backprojection3d_batched(const pyronn_cone_problem *p
   PYRONN_CHECK_LAUNCH("cone_bp_persistent_kernel");
   return rc;
 }
+

@@ -36,6 +36,26 @@ class _Geo:
         self.mult = _f32([g.projection_multiplier], dev)
 
 
+def _validate(inputs: Sequence[torch.Tensor], geometries: Sequence[geo.ConeGeometry], forward: bool):
+    """The kernels index each input with the shapes of its geometry: refuse any tensor that
+    is not a CUDA tensor on the first input's device with exactly those shapes."""
+    if len(inputs) != len(geometries):
+        raise ValueError(f"{len(inputs)} inputs for {len(geometries)} geometries")
+    _check(len(inputs))
+    dev = inputs[0].device
+    for k, (x, g) in enumerate(zip(inputs, geometries)):
+        if not isinstance(x, torch.Tensor) or not x.is_cuda:
+            raise ValueError(f"input {k} is not a CUDA tensor")
+        if x.device != dev:
+            raise ValueError(f"input {k} is on {x.device}, input 0 on {dev}")
+        want = tuple(g.volume_shape) if forward else (g.n_views,) + tuple(g.detector_shape)
+        if tuple(x.shape) != want:
+            what = "volume_shape" if forward else "(n_views,) + detector_shape"
+            raise ValueError(f"input {k} has shape {tuple(x.shape)}, its geometry's {what} is {want}")
+        if x.dtype not in _DT:
+            raise TypeError(f"input {k}: storage dtype {x.dtype} not supported")
+
+
 def _problems(inputs, outputs, geos: List[_Geo]):
     arr = (nat.ConeProblem * len(inputs))()
     for k, (x, y, gg) in enumerate(zip(inputs, outputs, geos)):
@@ -55,7 +75,7 @@ def _check(n):
 def cone_projection3d_batched(volumes: Sequence[torch.Tensor], geometries: Sequence[geo.ConeGeometry],
                               hardware_interp: bool = False, blocks_per_cu: int = 0) -> List[torch.Tensor]:
     """volumes[k] [D_k, H_k, W_k] (one storage dtype for all) -> sinograms [n_k, H_d,k, W_d,k] float32."""
-    _check(len(volumes))
+    _validate(volumes, geometries, True)
     dev = volumes[0].device
     dt = volumes[0].dtype
     if any(v.dtype != dt for v in volumes):
@@ -77,7 +97,7 @@ def cone_backprojection3d_batched(sinograms: Sequence[torch.Tensor], geometries:
                                   hardware_interp: bool = False, z_per_lane: int = 0,
                                   blocks_per_cu: int = 0) -> List[torch.Tensor]:
     """sinograms[k] [n_k, H_d,k, W_d,k] -> volumes [D_k, H_k, W_k] float32 (scaled by projection_multiplier)."""
-    _check(len(sinograms))
+    _validate(sinograms, geometries, False)
     dev = sinograms[0].device
     dt = sinograms[0].dtype
     if any(s.dtype != dt for s in sinograms):

@@ -49,6 +49,14 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def _cuda_device(device) -> torch.device:
+    """The plan's device with its index filled in, so that tensors can be compared against it."""
+    d = torch.device(device)
+    if d.type == "cuda" and d.index is None:
+        d = torch.device("cuda", torch.cuda.current_device())
+    return d
+
+
 class _Plan:
     storage: torch.dtype
     device: torch.device
@@ -81,6 +89,13 @@ class _Plan:
         return self.ramp
 
     def _filter(self, sino: torch.Tensor, images: int, det_h: int, det_w: int) -> torch.Tensor:
+        # the cosine weights (det_h x det_w) and the filter table were built for the plan's
+        # detector: a sinogram of any other row length or row count would read past them
+        if det_w != self._det_w or (self.weights is not None and self.weights.numel() != det_h * det_w):
+            raise ValueError(f"sinogram rows of {det_h} x {det_w} samples; the plan was built for "
+                             f"{tuple(self.weights.shape) if self.weights is not None else self._det_w}")
+        if not sino.is_cuda or sino.device != self.device:
+            raise ValueError(f"sinogram on {sino.device}, the plan on {self.device}")
         sino = sino.contiguous()
         code = _DT.get(sino.dtype)
         if code is None:
@@ -124,7 +139,7 @@ class ConeFDK(_Plan):
     def __init__(self, g: geo.ConeGeometry, device="cuda", storage: torch.dtype = torch.float32):
         _load_ops()
         self.g = g
-        self.device = torch.device(device)
+        self.device = _cuda_device(device)
         self.storage = storage
         n = g.n_views
         Hd, Wd = g.detector_shape
@@ -163,6 +178,13 @@ class ConeFDK(_Plan):
         Returns the workspace (holding q's detector cells) for a following call with bit 2."""
         _, k, Hd, Wd = q.shape
         D, H, W = self.g.volume_shape
+        if q.dim() != 4 or q.shape[0] != 1 or (Hd, Wd) != tuple(self.g.detector_shape):
+            raise ValueError(f"filtered views {tuple(q.shape)}; expected [1, k, {self.g.detector_shape[0]}, "
+                             f"{self.g.detector_shape[1]}]")
+        if not 0 <= view0 or view0 + k > self.g.n_views:
+            raise ValueError(f"views {view0} .. {view0 + k - 1} outside the orbit's {self.g.n_views} views")
+        if tuple(out.shape) != (1, D, H, W) or out.dtype != torch.float32 or not out.is_contiguous():
+            raise ValueError(f"out must be a contiguous float32 [1, {D}, {H}, {W}] tensor")
         z0, z1 = z if z is not None else (0, D)
         code = _DT[q.dtype]
         ws_bytes = nat.lib().pyronn_cone_backprojection3d_workspace_size(1, k, Hd, Wd, code)
@@ -236,7 +258,7 @@ class FanFBP(_Plan):
     def __init__(self, g: geo.Geometry2D, device="cuda", storage: torch.dtype = torch.float32):
         _load_ops()
         self.g = g
-        self.device = torch.device(device)
+        self.device = _cuda_device(device)
         self.storage = storage
         dev = self.device
         det, ds = g.detector_shape, g.detector_spacing
@@ -259,12 +281,16 @@ class FanFBP(_Plan):
         self._init_filter(det, ds * g.sid / g.sdd, colw)
         torch.cuda.current_stream().synchronize()
 
-    def __call__(self, sino: torch.Tensor) -> torch.Tensor:
-        """sino [B, n, det] -> image [B, H, W] float32."""
+    def filter(self, sino: torch.Tensor) -> torch.Tensor:
+        """sino [B, n, det] -> weighted, ramp-filtered rows (plan storage dtype)."""
         if sino.dim() == 2:
             sino = sino[None]
         B, n, det = sino.shape
-        q = self._filter(sino, B * n, 1, det)
+        return self._filter(sino, B * n, 1, det)
+
+    def __call__(self, sino: torch.Tensor) -> torch.Tensor:
+        """sino [B, n, det] -> image [B, H, W] float32."""
+        q = self.filter(sino)
         return torch.ops.pyronn_layers.fan_backprojection2d(q, list(self.g.volume_shape), self.vo, self.do,
                                                             self.vs, self.ds, self.sid, self.sdd, self.rays)
 
@@ -275,7 +301,7 @@ class ParallelFBP(_Plan):
     def __init__(self, g: geo.Geometry2D, device="cuda", storage: torch.dtype = torch.float32):
         _load_ops()
         self.g = g
-        self.device = torch.device(device)
+        self.device = _cuda_device(device)
         self.storage = storage
         dev = self.device
         det, ds = g.detector_shape, g.detector_spacing
@@ -291,10 +317,14 @@ class ParallelFBP(_Plan):
         self._init_filter(det, ds)
         torch.cuda.current_stream().synchronize()
 
-    def __call__(self, sino: torch.Tensor) -> torch.Tensor:
+    def filter(self, sino: torch.Tensor) -> torch.Tensor:
+        """sino [B, n, det] -> ramp-filtered rows (plan storage dtype)."""
         if sino.dim() == 2:
             sino = sino[None]
         B, n, det = sino.shape
-        q = self._filter(sino, B * n, 1, det)
+        return self._filter(sino, B * n, 1, det)
+
+    def __call__(self, sino: torch.Tensor) -> torch.Tensor:
+        q = self.filter(sino)
         return torch.ops.pyronn_layers.parallel_backprojection2d(q, list(self.g.volume_shape), self.vo, self.do,
                                                                  self.vs, self.ds, self.rays)

@@ -97,19 +97,61 @@ def test_view_range_backprojection_validation():
     assert rc == 1 and "null" in nat.last_error()
 
 
+def _fwd_rc(L, D, H, W, n=2, det=(4, 4), ws=16, batch=1):
+    return L.pyronn_cone_projection3d(_fake(), 0, batch, D, H, W, _fake(), _fake(), _fake(), _fake(), _fake(), n,
+                                      det[0], det[1], 0, _fake(), _fake(), ws, None)
+
+
+def _bp_rc(L, D, H, W, n=2, det=(4, 4), ws=16, batch=1):
+    return L.pyronn_cone_backprojection3d(_fake(), 0, batch, n, det[0], det[1], _fake(), _fake(), _fake(), _fake(),
+                                          _fake(), D, H, W, 0, _fake(), _fake(), ws, None)
+
+
+# Volumes of fewer than 2^32 voxels (the reference's uint32 index limit,
+# cc/kernels/cone_projector_3D_CudaKernel.cu.cc:52-59): cubes, slabs, rods, planes and
+# degenerate shapes, each with every side below 2^24 - 3.
+ACCEPTED_SHAPES = [(1625, 1625, 1625), (64, 32768, 8), (8, 32768, 64), (2048, 2048, 1023), (1, 1, (1 << 24) - 4),
+                   ((1 << 24) - 4, 1, 1), (1, (1 << 24) - 4, 1), (1, 65535, 65535), (65535, 65535, 1),
+                   (256, 4096, 4095), (1287, 1287, 1287), (1300, 1300, 1300)]
+
+
+@pytest.mark.parametrize("shape", ACCEPTED_SHAPES)
+def test_volumes_below_2_32_voxels_are_accepted(shape):
+    """Every check up to the workspace size passes (status 2 = workspace too small, with a
+    16-byte workspace): no PYRONN_ERR_TOO_LARGE for any volume the reference accepts."""
+    D, H, W = shape
+    assert D * H * W < 2 ** 32
+    L = nat.lib()
+    assert _fwd_rc(L, D, H, W) == 2, nat.last_error()
+    assert "workspace" in nat.last_error()
+    assert _bp_rc(L, D, H, W) == 2, nat.last_error()
+    assert "workspace" in nat.last_error()
+    # the workspace sizes are computed in 64-bit: (W+2)(H+2)(D+3) xy quads for the forward
+    need = L.pyronn_cone_projection3d_workspace_size(1, D, H, W, 2, 0)
+    assert need >= (W + 2) * (H + 2) * (D + 3) * 16
+
+
+def test_views_and_batches_beyond_one_grid_are_accepted():
+    """A 2048x2048 detector needs 16384 workgroups per view; 2000 views (3.3e7 workgroups)
+    exceed one dispatch's 2^32 work-items and run as several launches."""
+    L = nat.lib()
+    assert _fwd_rc(L, 64, 64, 64, n=2000, det=(2048, 2048)) == 2, nat.last_error()
+    assert _bp_rc(L, 64, 64, 64, n=2000, det=(2048, 2048)) == 2, nat.last_error()
+    # batch x depth beyond 65535 z blocks: split into launches
+    assert _bp_rc(L, 4096, 16, 16, batch=300) == 2, nat.last_error()
+
+
 def test_too_large_volume_rejected():
     L = nat.lib()
-    rc = L.pyronn_cone_projection3d(_fake(), 0, 1, 2048, 2048, 2048, _fake(), _fake(), _fake(), _fake(), _fake(), 2,
-                                    4, 4, 0, _fake(), _fake(), 1 << 62, None)
-    assert rc == 3
-    # few cells, but a row longer than the sampler's 24-bit index arithmetic allows
-    rc = L.pyronn_cone_projection3d(_fake(), 0, 1, 1, 1, (1 << 23) - 1, _fake(), _fake(), _fake(), _fake(), _fake(),
-                                    2, 4, 4, 0, _fake(), _fake(), 1 << 62, None)
-    assert rc == 3 and "index limits" in nat.last_error()
-    # the largest row that fits is accepted up to the workspace check
-    rc = L.pyronn_cone_projection3d(_fake(), 0, 1, 1, 1, (1 << 23) - 3, _fake(), _fake(), _fake(), _fake(), _fake(),
-                                    2, 4, 4, 0, _fake(), _fake(), 16, None)
-    assert rc == 2
+    # a side of 2^24 - 3 voxels or more (coordinates would no longer be exact floats)
+    assert _fwd_rc(L, 1, 1, (1 << 24) - 3, ws=1 << 62) == 3
+    assert "2^24" in nat.last_error()
+    assert _fwd_rc(L, (1 << 24) + 5, 1, 1, ws=1 << 62) == 3
+    # a detector image beyond the backprojection's 31-bit cell offsets
+    assert _bp_rc(L, 4, 4, 4, det=(1 << 15, 1 << 15), ws=1 << 62) == 3
+    assert "detector" in nat.last_error()
+    # batch x views beyond int
+    assert _fwd_rc(L, 4, 4, 4, n=1 << 20, batch=4096, ws=1 << 62) == 1
 
 
 def test_workspace_sizes_scale_with_storage():
@@ -120,12 +162,12 @@ def test_workspace_sizes_scale_with_storage():
     # per view: (H_d + 2) x (W_d + 2) cells of 4 neighbours
     assert L.pyronn_cone_backprojection3d_workspace_size(2, 10, 8, 8, 0) == \
         ((2 * 10 * 10 * 10 * 4 * 4 + 255) // 256) * 256
-    # geometry (12 floats per view) + cells ((W+2)(H+2)(D+3) xy quads) + z-fastest transpose of the volume
+    # geometry (12 floats per view) + cells ((W+2)(H+2)(D+3) xy quads); no other copy of the volume
     assert L.pyronn_cone_projection3d_workspace_size(1, 4, 4, 4, 2, 1) == \
-        ((1 * 2 * 12 * 4 + 255) // 256) * 256 + ((6 * 6 * 7 * 4 * 2 + 255) // 256) * 256 + 256
+        ((1 * 2 * 12 * 4 + 255) // 256) * 256 + ((6 * 6 * 7 * 4 * 2 + 255) // 256) * 256
     # the 16-bit copy is half the bytes of the f32 one
     assert L.pyronn_cone_projection3d_workspace_size(1, 4, 4, 4, 2, 0) == \
-        ((1 * 2 * 12 * 4 + 255) // 256) * 256 + ((6 * 6 * 7 * 4 * 4 + 255) // 256) * 256 + 256
+        ((1 * 2 * 12 * 4 + 255) // 256) * 256 + ((6 * 6 * 7 * 4 * 4 + 255) // 256) * 256
 
 
 def test_torch_ops_registered_and_cpu_tensors_rejected():

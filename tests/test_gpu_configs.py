@@ -83,12 +83,12 @@ def _fwd(vol_t, P, sp, og, det, step, hw=False):
         torch.tensor(P[None], device=dev), torch.full((1, 1), step, device=dev), torch.ones((1, 1), device=dev), hw)[0]
 
 
-def _bp(sino_t, P, sp, og, shape):
+def _bp(sino_t, P, sp, og, shape, hw=False):
     dev = sino_t.device
     return torch.ops.pyronn_layers.cone_backprojection3d(
         sino_t[None], list(shape), torch.tensor(og[None], device=dev), torch.tensor(sp[None], device=dev),
         torch.tensor(P[None], device=dev), torch.full((1, 1), 0.5, device=dev), torch.ones((1, 1), device=dev),
-        False)[0]
+        hw)[0]
 
 
 @pytest.mark.parametrize("hw", [False, True])
@@ -102,10 +102,15 @@ def test_c3_cone_256_views_and_slices(hw):
     for v0 in (0, 97):
         r, _, _ = orc.cone_forward(vh, inv, src, det, sp, 0.5, hw=hw, views=(v0, v0 + 1))
         gh.parity(sino[v0:v0 + 1].cpu().numpy(), r)
-    rec = _bp(sino, P, sp, og, shape)
+    # backprojection of the whole 360-view sinogram in the same interpolation mode
+    # (hardware_interp: cone_backprojector_3D_CudaKernel_hardware_interp.cu.cc:55-87)
+    rec = _bp(sino, P, sp, og, shape, hw)
     sh = sino.cpu().numpy()
-    rb, _ = orc.cone_backward(sh, P, shape, sp, og, hw=False, zrange=(128, 130))
-    gh.parity(rec[128:130].cpu().numpy(), rb)
+    for z0 in (0, 128, 254):
+        rb, _ = orc.cone_backward(sh, P, shape, sp, og, hw=hw, zrange=(z0, z0 + 2))
+        got = rec[z0:z0 + 2].cpu().numpy()
+        gh.parity(got, rb)
+        assert np.array_equal(got, rb), f"slices {z0}..{z0 + 1} hw={hw}: not bit-identical to the oracle"
 
 
 def test_m1_cone_512_subsets_and_properties():
