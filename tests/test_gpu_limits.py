@@ -97,7 +97,9 @@ def test_forward_64bit_cell_index_wide_plane(storage, hw):
     shape = (2048, 4096, 2)
     assert (shape[1] + 2) * (shape[0] + 3) >= 2 ** 23
     det = (128, 128)
-    sp, og, P = _orbit(shape, 4, 5000.0, 8000.0, det, (40.0, 40.0))
+    # 0.5 mm columns (0.31 mm at the isocentre), so that view 1, whose rays run along y,
+    # still has columns whose rays cross the 2-voxel-wide x extent
+    sp, og, P = _orbit(shape, 4, 5000.0, 8000.0, det, (40.0, 0.5))
     print(f"64-bit index {shape} {storage} hw={hw}", flush=True)
     vol = _rand_volume(shape, 2, storage)
     _check_views(vol, P, sp, og, det, 0.5, hw, views=(0, 1))
