@@ -98,6 +98,26 @@ int pyronn_cone_projection3d(const void *volume, int volume_dtype, int batch, in
                              int hardware_interp, float *sinogram, void *workspace,
                              size_t workspace_bytes, pyronn_stream_t stream);
 
+/*
+ * pyronn_cone_projection3d without a workspace: it allocates no device memory and takes
+ * none from the caller.  No counterpart in the reference; it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results bit for bit those of pyronn_cone_projection3d.
+ * Its only scratch is the per-view geometry, held in a fixed device array of 3 MiB per
+ * GPU (kept until the process ends); the volume is sampled in place rather than through a
+ * cell copy, so it runs slower than pyronn_cone_projection3d.
+ * Limit: batch * n_views <= 65536, else PYRONN_ERR_TOO_LARGE.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (on different streams, or from
+ * different threads) race and must be serialised by the caller.  The stream must belong
+ * to the current device.
+ */
+int pyronn_cone_projection3d_noalloc(const void *volume, int volume_dtype, int batch, int D, int H,
+                                     int W, const float *volume_origin, const float *volume_spacing,
+                                     const float *projection_matrices, const float *step_size,
+                                     const float *projection_multiplier, int n_views, int det_h,
+                                     int det_w, int hardware_interp, float *sinogram,
+                                     pyronn_stream_t stream);
+
 /* Workspace bytes for pyronn_cone_backprojection3d with these arguments. */
 size_t pyronn_cone_backprojection3d_workspace_size(int batch, int n_views, int det_h, int det_w,
                                                    int sinogram_dtype);

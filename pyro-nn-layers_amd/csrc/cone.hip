@@ -13,6 +13,7 @@
 // operation by operation (see the citations at each kernel), so that results
 // agree with the host restatement to the last bit.
 #include "common.h"
+#include <mutex>
 #include <type_traits>
 
 namespace pyronn {
@@ -269,7 +270,50 @@ __device__ __forceinline__ float sample_cells(const TS *__restrict__ cells, IT D
   return lerp_ref(p0, p1, xd);
 }
 
-template <typename TS, bool HW, bool CLAMP, typename IT>
+// Trilinear sample read straight from the volume [D][H][W], with no cell copy (the
+// forward variant that allocates nothing).  The eight neighbours of the cell that
+// sample_cells would read are gathered with a bounds test each (zero outside the volume,
+// as the cell array's border holds) and blended in the same order, so the two samplers
+// give the same bits for every coordinate.  W and HW are the row and plane strides.
+template <typename TS, bool CLAMP>
+__device__ __forceinline__ float sample_direct(const TS *__restrict__ vol, long long W, long long HW,
+                                               float px, float py, float pz, float fW, float fH,
+                                               float fD) {
+  if (CLAMP) {
+    px = clamp_apron(px, fW);
+    py = clamp_apron(py, fH);
+    pz = clamp_apron(pz, fD);
+  }
+  const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+  const float xd = px - fx, yd = py - fy, zd = pz - fz;
+  // the floors lie in [-1, N] (clamped, or proven by the caller): exact small integers
+  const bool x0 = fx >= 0.0f && fx < fW, x1 = fx >= -1.0f && fx + 1.0f < fW;
+  const bool y0 = fy >= 0.0f && fy < fH, y1 = fy >= -1.0f && fy + 1.0f < fH;
+  const bool z0 = fz >= 0.0f && fz < fD, z1 = fz >= -1.0f && fz + 1.0f < fD;
+  const long long base = (long long)(int)fz * HW + (long long)(int)fy * W + (long long)(int)fx;
+  const TS *p = vol + base;
+  const float zero = 0.0f;
+  // corner digits are z, y, x, as in sample_cells
+  const float c000 = (z0 && y0 && x0) ? to_f32(p[0]) : zero;
+  const float c001 = (z0 && y0 && x1) ? to_f32(p[1]) : zero;
+  const float c010 = (z0 && y1 && x0) ? to_f32(p[W]) : zero;
+  const float c011 = (z0 && y1 && x1) ? to_f32(p[W + 1]) : zero;
+  const float c100 = (z1 && y0 && x0) ? to_f32(p[HW]) : zero;
+  const float c101 = (z1 && y0 && x1) ? to_f32(p[HW + 1]) : zero;
+  const float c110 = (z1 && y1 && x0) ? to_f32(p[HW + W]) : zero;
+  const float c111 = (z1 && y1 && x1) ? to_f32(p[HW + W + 1]) : zero;
+  const float p00 = lerp_ref(c000, c100, zd);
+  const float p01 = lerp_ref(c001, c101, zd);
+  const float p10 = lerp_ref(c010, c110, zd);
+  const float p11 = lerp_ref(c011, c111, zd);
+  const float p0 = lerp_ref(p00, p10, yd);
+  const float p1 = lerp_ref(p01, p11, yd);
+  return lerp_ref(p0, p1, xd);
+}
+
+// DIRECT: c is the volume itself and Dc, HDc its row and plane strides (sample_direct);
+// otherwise c is the cell array (sample_cells).
+template <typename TS, bool HW, bool CLAMP, typename IT, bool DIRECT>
 __device__ __forceinline__ float cone_sample(const TS *__restrict__ c, IT Dc, IT HDc, float px,
                                              float py, float pz, float fW, float fH, float fD,
                                              CellRegs<TS, IT> &r) {
@@ -278,14 +322,17 @@ __device__ __forceinline__ float cone_sample(const TS *__restrict__ c, IT Dc, IT
     py = (py + 0.5f) - 0.5f;
     pz = (pz + 0.5f) - 0.5f;
   }
-  return sample_cells<TS, CLAMP, IT>(c, Dc, HDc, px, py, pz, fW, fH, fD, r);
+  if constexpr (DIRECT)
+    return sample_direct<TS, CLAMP>(c, (long long)Dc, (long long)HDc, px, py, pz, fW, fH, fD);
+  else
+    return sample_cells<TS, CLAMP, IT>(c, Dc, HDc, px, py, pz, fW, fH, fD, r);
 }
 
 // The march of one ray from its clipped parameter interval: entrance sample with weight
 // 1/2, loop samples, and the tail rule (cone_projector_3D_CudaKernel.cu.cc:134-179).
 // GUARDED = false drops the coordinate clamp and the iteration cap; the caller uses it
 // only for rays where neither can change anything (see fwd_block).
-template <typename TS, bool HW, bool GUARDED, typename IT>
+template <typename TS, bool HW, bool GUARDED, typename IT, bool DIRECT>
 __device__ __forceinline__ float march_ray(const TS *__restrict__ v, IT Dc, IT HDc, float fW,
                                            float fH, float fD, float rx, float ry, float rz,
                                            float s0, float s1, float s2, float amin, float amax,
@@ -297,19 +344,19 @@ __device__ __forceinline__ float march_ray(const TS *__restrict__ v, IT Dc, IT H
     px = fmaf(amin, rx, s0);
     py = fmaf(amin, ry, s1);
     pz = fmaf(amin, rz, s2);
-    pix = fmaf(0.5f, cone_sample<TS, HW, GUARDED, IT>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs), pix);
+    pix = fmaf(0.5f, cone_sample<TS, HW, GUARDED, IT, DIRECT>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs), pix);
     amin += step;
   }
   for (int it = 0; amin < amax && (!GUARDED || it < cap); ++it) {
     px = fmaf(amin, rx, s0);
     py = fmaf(amin, ry, s1);
     pz = fmaf(amin, rz, s2);
-    pix += cone_sample<TS, HW, GUARDED, IT>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs);
+    pix += cone_sample<TS, HW, GUARDED, IT, DIRECT>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs);
     amin += step;
   }
   pix *= step;
   if (pix > 0.0f) {  // tail only for a positive sum (cone_projector_3D_CudaKernel.cu.cc:164-179)
-    const float flast = cone_sample<TS, HW, GUARDED, IT>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs);
+    const float flast = cone_sample<TS, HW, GUARDED, IT, DIRECT>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs);
     pix = fmaf(-(0.5f * step), flast, pix);
     amin -= step;
     const float last = amax - amin;
@@ -317,7 +364,7 @@ __device__ __forceinline__ float march_ray(const TS *__restrict__ v, IT Dc, IT H
     px = fmaf(amax, rx, s0);
     py = fmaf(amax, ry, s1);
     pz = fmaf(amax, rz, s2);
-    pix = fmaf(0.5f * last, cone_sample<TS, HW, GUARDED, IT>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs),
+    pix = fmaf(0.5f * last, cone_sample<TS, HW, GUARDED, IT, DIRECT>(v, Dc, HDc, px, py, pz, fW, fH, fD, regs),
                pix);
   }
   return pix;
@@ -355,7 +402,8 @@ struct FwdProblem {
   long long begin;       // first work item of the problem in a batched launch (multiple of 8)
 };
 
-template <typename TS, bool HW, typename IT>
+// DIRECT: P.cells is the volume [D][H][W] itself, sampled with sample_direct.
+template <typename TS, bool HW, typename IT, bool DIRECT = false>
 __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int xcd = (int)(q % kXcds);
@@ -377,7 +425,9 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   const float step = P.step[0];
   const float spx = P.spacing[2], spy = P.spacing[1], spz = P.spacing[0];
   const TS *v = static_cast<const TS *>(P.cells);
-  const IT Wp = (IT)D + CellLayout<TS>::kZExtra, HWp = (IT)(H + 2) * Wp;  // z-fastest cell strides
+  // z-fastest cell strides, or the volume's row and plane strides (DIRECT)
+  const IT Wp = DIRECT ? (IT)W : (IT)D + CellLayout<TS>::kZExtra;
+  const IT HWp = DIRECT ? (IT)H * W : (IT)(H + 2) * Wp;
   const float fW = (float)W, fH = (float)H, fD = (float)D;
   float *out = P.sino + ((size_t)view * Hd + row) * (size_t)Wd + col;
   const float u = (float)col, w = (float)row;
@@ -434,16 +484,17 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
                      fmaxf(fabsf(s0), fmaxf(fabsf(s1), fabsf(s2))) < 0x1p18f && amag < 0x1p19f &&
                      amag * 0x1p-20f <= step;
   const float pix =
-      __all(plain) ? march_ray<TS, HW, false, IT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step)
-                   : march_ray<TS, HW, true, IT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step);
+      __all(plain) ? march_ray<TS, HW, false, IT, DIRECT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step)
+                   : march_ray<TS, HW, true, IT, DIRECT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step);
   const float a = rx * spx, bq = ry * spy, c = rz * spz;
   *out = pix * sqrtf(fmaf(c, c, fmaf(bq, bq, a * a)));
 }
 
 // One launch covers the views P.n of a range of each item's n_all views (the host splits
 // the views so that a launch stays within the dispatch grid's 2^32 work-items); P.sino,
-// P.inv_ar and P.src point at the range's first view of item 0.
-template <typename TS, bool HW, typename IT>
+// P.inv_ar and P.src point at the range's first view of item 0.  cell_stride is the
+// elements per batch item of P.cells (the volume's with DIRECT).
+template <typename TS, bool HW, typename IT, bool DIRECT = false>
 __global__ void __launch_bounds__(256)
 cone_fwd_kernel(FwdProblem P, long long cell_stride, int n_all) {
   // blockIdx.z selects the batch item; items share shapes and have their own geometry
@@ -454,7 +505,7 @@ cone_fwd_kernel(FwdProblem P, long long cell_stride, int n_all) {
   P.src += (size_t)b * n_all * 3;
   P.step += b;
   P.spacing += (size_t)b * 3;
-  fwd_block<TS, HW, IT>(P, blockIdx.x);
+  fwd_block<TS, HW, IT, DIRECT>(P, blockIdx.x);
 }
 
 // Up to this many problems of different shapes run in one persistent launch.
@@ -840,6 +891,26 @@ static void launch_fwd(bool hw, const void *cells, float *sino, const float *inv
   }
 }
 
+// The forward that reads the volume directly (sample_direct, no cell copy) with a 64-bit
+// volume index; the same views per launch and block order as launch_fwd.
+template <typename TS>
+static void launch_fwd_direct(bool hw, const void *vol, float *sino, const float *inv_ar,
+                              const float *src, const float *step, const float *spacing, int B, int D,
+                              int H, int W, int n, int Hd, int Wd, hipStream_t stream) {
+  const long long vs = (long long)D * H * W;
+  const int per = fwd_views_per_launch(n, Hd, Wd);
+  for (int v0 = 0; v0 < n; v0 += per) {
+    const int k = n - v0 < per ? n - v0 : per;
+    FwdProblem P{vol, sino + (size_t)v0 * Hd * Wd, inv_ar + (size_t)v0 * 9, src + (size_t)v0 * 3, step,
+                 spacing, W, H, D, k, Hd, Wd, (Wd + 15) / 16, (Hd + 15) / 16, 0};
+    dim3 grid((unsigned)fwd_blocks(k, Hd, Wd), 1, B);
+    if (hw)
+      cone_fwd_kernel<TS, true, long long, true><<<grid, dim3(256), 0, stream>>>(P, vs, n);
+    else
+      cone_fwd_kernel<TS, false, long long, true><<<grid, dim3(256), 0, stream>>>(P, vs, n);
+  }
+}
+
 // z voxels per lane: 8 for f32 detector cells, 4 for the 16-bit ones (measured
 // on the 512^3 / 720-view / 1024x768 workload).
 template <typename TS> constexpr int bp_zpt() { return sizeof(TS) == 4 ? 8 : 4; }
@@ -925,6 +996,31 @@ extern "C" size_t pyronn_cone_projection3d_workspace_size(int batch, int D, int 
          align_up((size_t)batch * cell_elems_dt(volume_dtype, D, H, W) * dtype_bytes(volume_dtype), 256);
 }
 
+// Argument checks shared by the forward entry points.
+static int fwd_check_args(const char *name, const void *volume, int volume_dtype, int batch, int D, int H,
+                          int W, const float *volume_origin, const float *volume_spacing,
+                          const float *projection_matrices, const float *step_size, int n_views,
+                          int det_h, int det_w, const float *sinogram) {
+  PYRONN_CHECK_ARG(volume && volume_origin && volume_spacing && projection_matrices && step_size && sinogram,
+                   "%s: null pointer", name);
+  PYRONN_CHECK_ARG(dtype_ok(volume_dtype), "%s: bad volume dtype %d", name, volume_dtype);
+  PYRONN_CHECK_ARG(batch > 0 && batch <= kMaxGridYZ && D > 0 && H > 0 && W > 0 && n_views > 0 &&
+                       det_h > 0 && det_w > 0,
+                   "%s: bad shape B=%d D=%d H=%d W=%d n=%d det=%dx%d", name, batch, D, H, W, n_views,
+                   det_h, det_w);
+  PYRONN_CHECK_ARG((long long)batch * n_views < (long long)INT32_MAX, "%s: batch x views %lld too large",
+                   name, (long long)batch * n_views);
+  if (!fwd_volume_ok(D, H, W)) {
+    set_error("%s: volume %dx%dx%d: a side of 2^24 - 3 voxels or more", name, D, H, W);
+    return PYRONN_ERR_TOO_LARGE;
+  }
+  if (fwd_views_per_launch(n_views, det_h, det_w) < 1) {
+    set_error("%s: detector %dx%d exceeds one launch", name, det_h, det_w);
+    return PYRONN_ERR_TOO_LARGE;
+  }
+  return PYRONN_OK;
+}
+
 extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, int batch, int D,
                                         int H, int W, const float *volume_origin,
                                         const float *volume_spacing,
@@ -935,24 +1031,10 @@ extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, in
                                         pyronn_stream_t stream) {
   (void)projection_multiplier;  // read but unused by the reference (cone_projector_3D_OPKernel.cc:213-215)
   clear_error();
-  PYRONN_CHECK_ARG(volume && volume_origin && volume_spacing && projection_matrices && step_size &&
-                       sinogram && workspace,
-                   "cone_projection3d: null pointer");
-  PYRONN_CHECK_ARG(dtype_ok(volume_dtype), "cone_projection3d: bad volume dtype %d", volume_dtype);
-  PYRONN_CHECK_ARG(batch > 0 && batch <= kMaxGridYZ && D > 0 && H > 0 && W > 0 && n_views > 0 &&
-                       det_h > 0 && det_w > 0,
-                   "cone_projection3d: bad shape B=%d D=%d H=%d W=%d n=%d det=%dx%d", batch, D, H, W,
-                   n_views, det_h, det_w);
-  PYRONN_CHECK_ARG((long long)batch * n_views < (long long)INT32_MAX,
-                   "cone_projection3d: batch x views %lld too large", (long long)batch * n_views);
-  if (!fwd_volume_ok(D, H, W)) {
-    set_error("cone_projection3d: volume %dx%dx%d: a side of 2^24 - 3 voxels or more", D, H, W);
-    return PYRONN_ERR_TOO_LARGE;
-  }
-  if (fwd_views_per_launch(n_views, det_h, det_w) < 1) {
-    set_error("cone_projection3d: detector %dx%d exceeds one launch", det_h, det_w);
-    return PYRONN_ERR_TOO_LARGE;
-  }
+  int rc = fwd_check_args("cone_projection3d", volume, volume_dtype, batch, D, H, W, volume_origin,
+                          volume_spacing, projection_matrices, step_size, n_views, det_h, det_w, sinogram);
+  if (rc != PYRONN_OK) return rc;
+  PYRONN_CHECK_ARG(workspace != nullptr, "cone_projection3d: null pointer");
   const size_t need = pyronn_cone_projection3d_workspace_size(batch, D, H, W, n_views, volume_dtype);
   if (workspace_bytes < need) {
     set_error("cone_projection3d: workspace %zu bytes < required %zu", workspace_bytes, need);
@@ -965,8 +1047,8 @@ extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, in
   float *srcp = inv_ar + (size_t)batch * n_views * 9;
   void *cells = ws + fwd_geom_bytes(batch, n_views);
 
-  int rc = pyronn_cone_geometry_setup(projection_matrices, volume_origin, volume_spacing, batch,
-                                      n_views, inv_ar, srcp, stream);
+  rc = pyronn_cone_geometry_setup(projection_matrices, volume_origin, volume_spacing, batch, n_views,
+                                  inv_ar, srcp, stream);
   if (rc != PYRONN_OK) return rc;
   const bool hw = hardware_interp != 0;
   if (volume_dtype == PYRONN_F32) {
@@ -983,6 +1065,82 @@ extern "C" int pyronn_cone_projection3d(const void *volume, int volume_dtype, in
                          W, n_views, det_h, det_w, s);
   }
   PYRONN_CHECK_LAUNCH("cone_fwd_kernel");
+  return PYRONN_OK;
+}
+
+// =====================================================================================
+// The forward projection that allocates nothing (for callers that cannot allocate device
+// memory: code under HIP graph capture, serving loops, harnesses that forbid it).  Its
+// only scratch is the per-view geometry, in a file-scope device array of fixed size; the
+// volume is sampled directly (sample_direct) instead of through a cell copy.
+// =====================================================================================
+namespace pyronn {
+constexpr int kNoAllocMaxViews = 1 << 16;  // batch x views of one call
+constexpr int kNoAllocMaxDevices = 64;
+// inv_ar [batch][n][9], then src [batch][n][3]: 3 MiB, held by each device the library
+// runs on until the process ends.
+static __device__ __align__(256) float scratch[(size_t)kNoAllocMaxViews * 12];
+
+// The device address of `scratch` on device `dev` (the current device), looked up once per
+// device.  In host code the array's name is only a host-side stand-in; kernels get this
+// address.
+static float *scratch_address(int dev, int &rc) {
+  static std::mutex mu;
+  static float *addr[kNoAllocMaxDevices] = {};
+  std::lock_guard<std::mutex> lock(mu);
+  if (addr[dev] == nullptr) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      rc = hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "cone_projection3d_noalloc: hipGetSymbolAddress");
+      return nullptr;
+    }
+    addr[dev] = static_cast<float *>(p);
+  }
+  return addr[dev];
+}
+}  // namespace pyronn
+
+extern "C" int pyronn_cone_projection3d_noalloc(const void *volume, int volume_dtype, int batch, int D,
+                                                int H, int W, const float *volume_origin,
+                                                const float *volume_spacing,
+                                                const float *projection_matrices,
+                                                const float *step_size,
+                                                const float *projection_multiplier, int n_views,
+                                                int det_h, int det_w, int hardware_interp,
+                                                float *sinogram, pyronn_stream_t stream) {
+  (void)projection_multiplier;  // read but unused by the reference (cone_projector_3D_OPKernel.cc:213-215)
+  clear_error();
+  int rc = fwd_check_args("cone_projection3d_noalloc", volume, volume_dtype, batch, D, H, W, volume_origin,
+                          volume_spacing, projection_matrices, step_size, n_views, det_h, det_w, sinogram);
+  if (rc != PYRONN_OK) return rc;
+  if ((long long)batch * n_views > kNoAllocMaxViews) {
+    set_error("cone_projection3d_noalloc: batch x views = %lld exceeds the scratch limit of %d",
+              (long long)batch * n_views, kNoAllocMaxViews);
+    return PYRONN_ERR_TOO_LARGE;
+  }
+  int dev = 0;
+  const hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "cone_projection3d_noalloc: hipGetDevice");
+  PYRONN_CHECK_ARG(dev >= 0 && dev < kNoAllocMaxDevices, "cone_projection3d_noalloc: device %d not supported", dev);
+  float *inv_ar = scratch_address(dev, rc);
+  if (inv_ar == nullptr) return rc;
+  float *srcp = inv_ar + (size_t)batch * n_views * 9;
+  rc = pyronn_cone_geometry_setup(projection_matrices, volume_origin, volume_spacing, batch, n_views,
+                                  inv_ar, srcp, stream);
+  if (rc != PYRONN_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const bool hw = hardware_interp != 0;
+  if (volume_dtype == PYRONN_F32)
+    launch_fwd_direct<float>(hw, volume, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H, W,
+                             n_views, det_h, det_w, s);
+  else if (volume_dtype == PYRONN_BF16)
+    launch_fwd_direct<bf16s>(hw, volume, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H, W,
+                             n_views, det_h, det_w, s);
+  else
+    launch_fwd_direct<_Float16>(hw, volume, sinogram, inv_ar, srcp, step_size, volume_spacing, batch, D, H,
+                                W, n_views, det_h, det_w, s);
+  PYRONN_CHECK_LAUNCH("cone_fwd_kernel (direct)");
   return PYRONN_OK;
 }
 

@@ -39,6 +39,8 @@ _SIGS = {
     "pyronn_cone_projection3d_workspace_size": (_sz, [_i, _i, _i, _i, _i, _i]),
     "pyronn_cone_projection3d": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp,
                                       _vp, _sz, _vp]),
+    "pyronn_cone_projection3d_noalloc": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i,
+                                              _vp, _vp]),
     "pyronn_cone_backprojection3d_workspace_size": (_sz, [_i, _i, _i, _i, _i]),
     "pyronn_cone_backprojection3d": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i,
                                           _vp, _vp, _sz, _vp]),

@@ -221,3 +221,18 @@ def test_fft_filter_limits_and_validation():
     assert L.pyronn_fdk_filter_fft(None, 0, 1, 4, 64, None, _fake(2), _fake(3), 0, None) == 1
     assert "null" in nat.last_error()
 
+
+
+def test_noalloc_forward_checks_arguments_before_any_device_call():
+    """The no-allocation forward rejects bad arguments and more views than its fixed scratch
+    holds (batch x views > 65536) on the host, before touching the device."""
+    L = nat.lib()
+    rc = L.pyronn_cone_projection3d_noalloc(_fake(), 0, 2, 4, 4, 4, _fake(), _fake(), _fake(), _fake(), _fake(),
+                                            40000, 4, 4, 0, _fake(), None)
+    assert rc == 3 and "scratch" in nat.last_error()
+    rc = L.pyronn_cone_projection3d_noalloc(None, 0, 1, 4, 4, 4, _fake(), _fake(), _fake(), _fake(), _fake(), 2, 4,
+                                            4, 0, _fake(), None)
+    assert rc == 1 and "null" in nat.last_error()
+    rc = L.pyronn_cone_projection3d_noalloc(_fake(), 5, 1, 4, 4, 4, _fake(), _fake(), _fake(), _fake(), _fake(), 2,
+                                            4, 4, 0, _fake(), None)
+    assert rc == 1 and "dtype" in nat.last_error()
