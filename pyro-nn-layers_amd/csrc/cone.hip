@@ -375,7 +375,11 @@ __device__ __forceinline__ float march_ray(const TS *__restrict__ v, IT Dc, IT H
 // 16 x 16 detector tile and each wave 4 columns x 16 rows of it.  For an orbit about z,
 // rays of one detector column stay at nearly the same (x, y) and differ in z, so with
 // z-fastest cells the 16 rows of a wave column share cache lines (measured 3% faster
-// than 8 x 8 waves at 512^3).
+// than 8 x 8 waves at 512^3).  Lanes are numbered rows fastest: the texture-address
+// unit takes a wave's load four lanes at a time and spends a cycle per distinct 128-byte
+// line within each group of four, so each group is four neighbouring rows of one column
+// (z-neighbours in one z-fastest cell column, mostly one line) rather than four columns
+// (up to four different cell columns).
 // Work order: workgroups are dispatched round-robin over the 8 XCDs, so block id
 // b runs on XCD b % 8.  XCD k takes detector tile rows k, k+8, k+16, ...; within a
 // tile row it walks tiles (outer) and views (inner).  The blocks resident on one XCD
@@ -415,8 +419,8 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   const long long w_ = qq - (long long)band * per_row;
   const int tx = (int)(w_ / P.n);
   const int view = (int)(w_ - (long long)tx * P.n);
-  const int col = tx * 16 + wave * 4 + (lane & 3);
-  const int row = ty * 16 + (lane >> 2);
+  const int col = tx * 16 + wave * 4 + (lane >> 4);
+  const int row = ty * 16 + (lane & 15);
   const int W = P.W, H = P.H, D = P.D, Hd = P.Hd, Wd = P.Wd;
   if (col >= Wd || row >= Hd) return;
 
