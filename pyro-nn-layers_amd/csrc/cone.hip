@@ -688,8 +688,13 @@ __device__ __forceinline__ bool bp_region_tile(int r, int tiles_x, int tiles_y, 
 
 template <typename TS, bool HW, int ZPT, bool FIDX>
 __device__ __forceinline__ void bp_block(const BpProblem &Q, int tx, int ty, int zb) {
-  const int i = tx * 16 + (threadIdx.x & 15);
-  const int j = ty * 16 + (threadIdx.x >> 4);
+  // Each group of four lanes (what the texture-address unit takes per cycle; it spends a
+  // cycle per distinct 128-byte line in the group) is a 2 x 2 voxel square, whose detector
+  // footprint is narrower than that of four voxels in a row for most view angles.  Lane
+  // bits: x0 y0 x1 x2 x3 y1, wave = y2 y3.
+  const int l = threadIdx.x & 63;
+  const int i = tx * 16 + ((l & 1) | (((l >> 2) & 7) << 1));
+  const int j = ty * 16 + (((l >> 1) & 1) | ((l >> 5) << 1) | ((threadIdx.x >> 6) << 2));
   const int k0 = Q.z0 + zb * ZPT;
   const int W = Q.W, H = Q.H, z1 = Q.z1, n = Q.n, Hd = Q.Hd, Wd = Q.Wd;
   if (i >= W || j >= H) return;
