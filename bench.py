@@ -133,10 +133,11 @@ def main(argv=None):
 
     medians = {}
 
-    def run(storage, hw=False):
+    def run(storage, hw=False, keep_rec=False):
         v = vol.to(storage)
         times_f, times_b = [], []
         sino = None
+        last_rec = None
         for it in range(args.warmup + args.steps):
             timed = it >= args.warmup
             e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
@@ -156,8 +157,12 @@ def main(argv=None):
             if timed:
                 times_f.append(tf)
                 times_b.append(tb)
+            if keep_rec:
+                last_rec = rec
             del rec
         medians[(storage, hw)] = (float(np.median(times_f)), float(np.median(times_b)))
+        if keep_rec:
+            return float(np.mean(times_f)), float(np.mean(times_b)), sino, last_rec
         return float(np.mean(times_f)), float(np.mean(times_b)), sino
 
     # Timed region: K steps bracketed by synchronize on both sides (inside run()).
@@ -177,12 +182,27 @@ def main(argv=None):
     dominant = "cone_fwd_kernel" if tf >= tb else "cone_bp_kernel"
     B_dom, t_dom = (B_fwd, tf) if tf >= tb else (B_bp, tb)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "cone_m1_f32_traffic.json")
-    if dominant == "cone_fwd_kernel" and os.path.exists(tpath):
-        # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
-        # scaled by the reading of a 1 GiB copy (see that file's source summary)
-        with open(tpath) as f:
-            traffic = json.load(f)["traffic_bytes"]
+    ceiling = None
+    cpath = os.path.join(ROOT, "profiles", "cone_m1_f32_counters.json")
+    if dominant == "cone_fwd_kernel" and os.path.exists(cpath):
+        # Counters of the same kernel on the same workload (rocprofv3 --pmc, one pass per
+        # counter group): HBM bytes per launch from FETCH_SIZE / WRITE_SIZE scaled by the
+        # reading of a 1 GiB copy, and the wave load instructions the address unit issues.
+        with open(cpath) as f:
+            ctr = json.load(f)
+        traffic = ctr["traffic_bytes"]
+        # The unit that binds: the texture-address path issues at most one wave load per 16
+        # cycles per CU (one cycle per group of four lanes; profiles/unit_rates_mi355x.md).
+        loads_per_cu_cycle = ctr["vmem_load_instructions"] / ctr["cus"] / (t_dom * ctr["shader_clock_hz"])
+        ceiling = {
+            "unit": "texture-address path (wave loads per CU per cycle)",
+            "achieved": round(loads_per_cu_cycle, 5), "peak": ctr["load_issue_peak_per_cu_cycle"],
+            "frac": round(loads_per_cu_cycle / ctr["load_issue_peak_per_cu_cycle"], 4),
+            "busy_frac": ctr["ta_busy_frac"],
+            "note": "the rest of the busy time is groups of four lanes reading two 128-byte lines",
+            "source": "profiles/cone_m1_f32_counters.json (SQ_INSTS_VMEM_RD, shader clock), "
+                      "profiles/unit_rates_mi355x.md (peak)",
+        }
     out = {
         "metric": METRIC,
         "value": round(gs, 3),
@@ -204,15 +224,15 @@ def main(argv=None):
         },
         "roofline": {
             "bound": "hbm", "kernel": dominant,
+            # gather-footprint model: the bytes the reference's gathers request (8 corners per
+            # sample), most of them served by the L1 and L2; a ratio above 1 means cache reuse
+            "model": "gather footprint (algorithmic bytes / HBM peak), not DRAM traffic",
             "achieved": round(B_dom / t_dom / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": round(B_dom / t_dom / HBM_PEAK, 4), "traffic": traffic,
-            "traffic_source": "profiles/cone_m1_f32_traffic.json (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
+            "traffic_source": "profiles/cone_m1_f32_counters.json (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
             # measured HBM bytes over the live duration: the share of HBM bandwidth actually used
             "traffic_frac": round(traffic / t_dom / HBM_PEAK, 4) if traffic else None,
-            "limiter": ("vector-memory address path (texture-address unit 99.9% busy, one L1 access per sample; "
-                        "profiles/cone_m1_counters.md): algorithmic bytes exceed HBM peak because the L1 serves "
-                        "~92% of corner reads and the L2 ~80% of the rest")
-            if dominant == "cone_fwd_kernel" else None,
+            "ceiling": ceiling,
         },
         "fwd_back": {
             "t_fwd_ms": round(tf * 1e3, 2), "t_bp_ms": round(tb * 1e3, 2),
@@ -229,8 +249,9 @@ def main(argv=None):
     if not args.no_bf16:
         del sino
         torch.cuda.empty_cache()
-        tf16, tb16, sino16 = run(torch.bfloat16)
-        del sino16
+        tf16, tb16, sino16, rec16 = run(torch.bfloat16, keep_rec=True)
+        prec16 = precision_vs_fp64(vol, sino16, rec16, torch.bfloat16, P, sp, og, cfg)
+        del sino16, rec16
         Bf = S_fwd * 8 * 2 + R * 4
         Bb = U_bp * 4 * 2 + V * 4
         g16 = (S_fwd + U_bp) / (tf16 + tb16) / 1e9
@@ -239,14 +260,16 @@ def main(argv=None):
             "t_fwd_ms": round(tf16 * 1e3, 2), "t_bp_ms": round(tb16 * 1e3, 2),
             "hbm_roofline_frac_fwd_back": round((Bf + Bb) / (tf16 + tb16) / HBM_PEAK, 4),
             "storage": "bf16 volume and sinogram, fp32 accumulation, fp32 outputs",
+            "error_vs_fp64": prec16,
         }
     else:
         del sino
 
     if not args.no_f16:
         torch.cuda.empty_cache()
-        tfh16, tbh16, sino_f16 = run(torch.float16)
-        del sino_f16
+        tfh16, tbh16, sino_f16, rec_f16 = run(torch.float16, keep_rec=True)
+        prec_f16 = precision_vs_fp64(vol, sino_f16, rec_f16, torch.float16, P, sp, og, cfg)
+        del sino_f16, rec_f16
         Bf = S_fwd * 8 * 2 + R * 4
         Bb = U_bp * 4 * 2 + V * 4
         gh16 = (S_fwd + U_bp) / (tfh16 + tbh16) / 1e9
@@ -255,6 +278,7 @@ def main(argv=None):
             "t_fwd_ms": round(tfh16 * 1e3, 2), "t_bp_ms": round(tbh16 * 1e3, 2),
             "hbm_roofline_frac_fwd_back": round((Bf + Bb) / (tfh16 + tbh16) / HBM_PEAK, 4),
             "storage": "fp16 volume and sinogram, fp32 accumulation, fp32 outputs",
+            "error_vs_fp64": prec_f16,
         }
 
     if not args.no_hw:
@@ -654,6 +678,74 @@ def batched_bench(args):
     return out
 
 
+def precision_vs_fp64(vol_t, sino16_t, rec16_t, storage, P, sp, og, cfg, views=(0, 180), slices=(128, 256)):
+    """Relative L2 errors of a 16-bit-storage run against the FP64 oracle, on two forward
+    views and two backprojected z slices of the 512^3 workload:
+    * kernel: FP64 fed the same dequantised input (the 16-bit volume; the 16-bit sinogram
+      the backprojection read);
+    * end_to_end: FP64 fed the input before rounding to 16 bits (the FP32 phantom; the
+      FP32 sinogram that the 16-bit forward produced)."""
+    import numpy as np
+    import torch
+
+    import oracle as orc
+
+    def rel(a, b):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    D, H, W = cfg["shape"]
+    t0 = time.perf_counter()
+    _, _, inv64, src64, _ = orc.cone_setup(P, sp, og)
+    v32 = vol_t[0].float().cpu().numpy()
+    vq = vol_t[0].to(storage).float().cpu().numpy()
+    got = sino16_t[0].cpu().numpy()
+    k_f, e_f = [], []
+    for v in views:
+        ref_q, _ = orc.cone_forward_f64(vq, inv64, src64, cfg["det"], sp, cfg["step"], views=(v, v + 1))
+        ref_e, _ = orc.cone_forward_f64(v32, inv64, src64, cfg["det"], sp, cfg["step"], views=(v, v + 1))
+        k_f.append(rel(got[v:v + 1], ref_q))
+        e_f.append(rel(got[v:v + 1], ref_e))
+    # the backprojection read sino16.to(storage): FP64 on that (kernel) and on the FP32 sinogram
+    s32 = sino16_t[0].cpu().numpy()
+    sq = sino16_t[0].to(storage).float().cpu().numpy()
+    rec = rec16_t[0].cpu().numpy()
+    k_b, e_b = [], []
+    for z in slices:
+        ref_q = orc.cone_backward_f64(sq, P, (D, H, W), sp, og, zrange=(z, z + 1))
+        ref_e = orc.cone_backward_f64(s32, P, (D, H, W), sp, og, zrange=(z, z + 1))
+        k_b.append(rel(rec[z:z + 1], ref_q))
+        e_b.append(rel(rec[z:z + 1], ref_e))
+    log(f"{storage} error vs fp64: forward kernel {max(k_f):.2e} end-to-end {max(e_f):.2e}; "
+        f"backprojection kernel {max(k_b):.2e} end-to-end {max(e_b):.2e} ({time.perf_counter() - t0:.1f} s)")
+    return {
+        "forward_views": list(views), "backprojection_slices": list(slices),
+        "forward_kernel_rel_l2": max(k_f), "forward_end_to_end_rel_l2": max(e_f),
+        "backprojection_kernel_rel_l2": max(k_b), "backprojection_end_to_end_rel_l2": max(e_b),
+        "bar": 1e-3,
+    }
+
+
+def host_cpu():
+    """Host description for the CPU baseline record."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"nproc": affinity, "os_cpu_count": os.cpu_count(), "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
+            "cpu_model": model}
+
+
 def cpu_baseline(vol_t, P, sp, og, cfg, args):
     """The f32 oracle (the reference kernel arithmetic on the host, OpenMP) on a
     bounded sample: `cpu_views` evenly spaced forward views and the backprojection
@@ -693,7 +785,7 @@ def cpu_baseline(vol_t, P, sp, og, cfg, args):
     U = args.cpu_slices * H * W * n
     return {
         "value": round((S + U) / (tf + tb) / 1e9, 4), "unit": "Gsamples/s", "cores": orc.num_threads(),
-        "kind": "port",
+        "kind": "port", **host_cpu(),
         "sample": f"f32 oracle: forward of {len(views)} of {n} views ({S} samples, {tf:.1f} s) + backprojection "
                   f"of {args.cpu_slices} of {D} z-slices over all views ({U} updates, {tb:.1f} s)",
     }

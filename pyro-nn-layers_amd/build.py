@@ -24,6 +24,8 @@ HIP_LIB = os.path.join(OUT_DIR, "libpyronn_mi.so")
 OPS_LIB = os.path.join(OUT_DIR, "libpyronn_torch_ops.so")
 TESTKIT_SRC = os.path.join(REPO, "tests", "native", "rcp_check.hip")
 TESTKIT_LIB = os.path.join(REPO, "tests", "native", "libpyronn_testkit.so")
+RATES_SRC = os.path.join(REPO, "tests", "native", "unit_rates.hip")
+RATES_BIN = os.path.join(REPO, "tests", "native", "unit_rates")
 
 HIP_SOURCES = ["abi_common.cpp", "cone.hip", "planar.hip", "fdk.hip"]
 HIP_FLAGS = [
@@ -79,6 +81,17 @@ def build_testkit(verbose=False) -> str:
     return TESTKIT_LIB
 
 
+def build_rates(verbose=False) -> str:
+    """The unit-rate measurement program (load and VALU issue rates per CU; tests/native)."""
+    tmp = RATES_BIN + ".tmp"
+    dt, _ = _run([hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                  "-packed-fp32-ops", RATES_SRC, "-o", tmp])
+    if verbose:
+        print(f"  unit_rates.hip: {dt:.1f}s")
+    os.replace(tmp, RATES_BIN)
+    return RATES_BIN
+
+
 def build_ops(verbose=False) -> str:
     """g++ build of the TORCH_LIBRARY op library against libtorch and libpyronn_mi."""
     import torch
@@ -110,6 +123,8 @@ def main(argv=None):
     print("building", build_hip(a.verbose))
     if os.path.exists(TESTKIT_SRC):
         print("building", build_testkit(a.verbose))
+    if os.path.exists(RATES_SRC):
+        print("building", build_rates(a.verbose))
     if not a.hip_only and os.path.exists(os.path.join(CSRC, "torch_ops.cpp")):
         print("building", build_ops(a.verbose))
 
