@@ -65,6 +65,10 @@ def main(argv=None):
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--serve-chunks", type=int, default=8,
                     help="view ranges of a 512^3 request uploaded while earlier ranges reconstruct (1 = off)")
+    ap.add_argument("--serve-curve", type=lambda t: [float(x) for x in t.split(",") if x], default=[16.0, 32.0, 48.0, 64.0],
+                    help="offered loads (req/s) of the serving latency curve, comma separated")
+    ap.add_argument("--no-serve-projections", dest="serve_projections", action="store_false",
+                    help="skip the serving run that mixes projection requests in")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-batch run")
     ap.add_argument("--batched-only", action="store_true")
@@ -401,27 +405,112 @@ def serve_bench(args):
         p512.ramp = None
         del x
         chunked = {"cone512": args.serve_chunks} if args.serve_chunks > 1 else {}
-        ex = GpuExecutor(plans, slots=args.serve_slots, use_graphs=True, chunked=chunked)
+        ex = GpuExecutor(plans, slots=args.serve_slots, use_graphs=True, chunked=chunked,
+                         group_kinds=("cone256",))
         log(f"serving {name}: capturing graphs")
         ex.warm()
-        reqs = poisson_arrivals(5, args.serve_rate, args.serve_requests, mix)
-        for r in reqs:
-            r.payload = payloads[r.kind][r.rid % 2]
-        log(f"serving {name}: {len(reqs)} requests")
-        res = Scheduler(ex, slots=args.serve_slots, hbm_budget=160 << 30).run(reqs, idle=lambda dt: time.sleep(
-            dt) if dt > 0 else None)
-        rep = latency_report(res)
+
+        def serve(rate, group_max=1, at_once=False):
+            reqs = poisson_arrivals(5, rate, args.serve_requests, mix)
+            for r in reqs:
+                r.payload = payloads[r.kind][r.rid % 2]
+                if at_once:
+                    r.arrival = 0.0
+            res = Scheduler(ex, slots=args.serve_slots, hbm_budget=160 << 30, group_max=group_max).run(
+                reqs, idle=lambda dt: time.sleep(dt) if dt > 0 else None)
+            return _rounded(latency_report(res))
+
+        log(f"serving {name}: {args.serve_requests} requests at {args.serve_rate} req/s")
+        rep = serve(args.serve_rate)
         log(f"serving {name}: {rep['req_per_s']:.2f} req/s p50 {rep['p50_ms']:.1f} ms p99 {rep['p99_ms']:.1f} ms")
-        out[name] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in rep.items()}
-        # capacity: the same requests, all queued at t = 0 (throughput limited by the GPU, not arrivals)
-        sat = [Request(r.rid, r.kind, 0.0, r.payload) for r in reqs]
-        res_s = Scheduler(ex, slots=args.serve_slots, hbm_budget=160 << 30).run(sat, idle=lambda dt: time.sleep(
-            dt) if dt > 0 else None)
-        rep_s = latency_report(res_s)
-        log(f"serving {name} saturated: {rep_s['req_per_s']:.2f} req/s")
-        out[name]["capacity"] = {k: round(rep_s[k], 3) for k in ("req_per_s", "span_s")}
-        out[name]["capacity"]["completed"] = rep_s["completed"]
+        out[name] = rep
+        # p50 / p99 against the offered load, up to saturation
+        curve = []
+        for rate in args.serve_curve:
+            r_ = serve(rate)
+            curve.append({"offered_req_per_s": rate, **{k: r_[k] for k in ("req_per_s", "p50_ms", "p99_ms")},
+                          "per_kind": r_["per_kind"]})
+            log(f"serving {name} at {rate} req/s: {r_['req_per_s']:.2f} req/s p50 {r_['p50_ms']:.1f} ms "
+                f"p99 {r_['p99_ms']:.1f} ms")
+        out[name]["curve"] = curve
+        # capacity: the same requests, all queued at t = 0 (throughput limited by the GPU, not arrivals),
+        # each request alone in its slot, then queued cone 256^3 requests grouped into one batched
+        # backprojection launch (up to 4 per group)
+        for key, gm in (("capacity", 1), ("capacity_grouped_cone256", 4)):
+            r_ = serve(args.serve_rate, group_max=gm, at_once=True)
+            log(f"serving {name} saturated, group_max {gm}: {r_['req_per_s']:.2f} req/s")
+            out[name][key] = {"req_per_s": r_["req_per_s"], "span_s": r_["span_s"], "completed": r_["completed"],
+                              "p99_ms": r_["p99_ms"], "group_max": gm}
         out[name]["filter_cone512"] = filt
+        del ex, plans, payloads
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    if args.serve_projections:
+        out["with_projection_requests"] = serve_mixed_bench(args, geoms, clean)
+    return out
+
+
+def _rounded(rep):
+    out = {}
+    for k, v in rep.items():
+        if isinstance(v, float):
+            out[k] = round(v, 3)
+        elif isinstance(v, dict):
+            out[k] = _rounded(v)
+        else:
+            out[k] = v
+    return out
+
+
+def serve_mixed_bench(args, geoms, clean):
+    """Reconstruction and projection requests in one Poisson stream: 30% fan 512^2 FBP,
+    30% cone 256^3 FDK, 10% cone 512^3 FDK (view ranges), 15% cone 256^3 forward projections
+    (volume in, 360 x 400 x 400 sinogram out) and 15% fan 512^2 forward projections
+    (360 x 725), every geometry resident on the device; FP32 and BF16 storage."""
+    import numpy as np
+    import torch
+
+    from pyronn_layers import phantoms as ph
+    from pyronn_layers.fdk import ConeFDK, FanFBP
+    from pyronn_layers.projectors import ConeProjector, FanProjector
+    from pyronn_layers.serving import GpuExecutor, Scheduler, latency_report, poisson_arrivals
+
+    dev = torch.device("cuda:0")
+    gc, gf = geoms["cone256"], geoms["fan512"]
+    e = ph.random_ellipsoids(12, 20, gc.volume_shape, gc.volume_spacing, axes_range_vox=(8.0, 100.0))
+    vol = ph.rasterise_ellipsoids_torch(e, gc.volume_shape, gc.volume_spacing, gc.volume_origin, device=dev)[None]
+    ef = ph.random_ellipses(11, 16, gf.volume_shape, gf.volume_spacing)
+    img = torch.tensor(ph.rasterise_ellipses(ef, gf.volume_shape, gf.volume_spacing, gf.volume_origin)[None],
+                       device=dev)
+    mix = {"fan512": 0.3, "cone256": 0.3, "cone512": 0.1, "cone256_proj": 0.15, "fan512_proj": 0.15}
+    out = {"workload": "Poisson arrivals (seed 9, %.1f req/s, %d requests): 30%% fan 512^2 FBP, 30%% cone 256^3 "
+                       "FDK, 10%% cone 512^3 FDK, 15%% cone 256^3 forward projections, 15%% fan 512^2 forward "
+                       "projections; inputs from pinned host memory, outputs back to host"
+                       % (args.serve_rate, args.serve_requests), "slots": args.serve_slots}
+    rng = np.random.default_rng(77)
+    for name, storage in (("f32", torch.float32), ("bf16", torch.bfloat16)):
+        plans = {"fan512": FanFBP(gf, storage=storage), "cone256": ConeFDK(gc, storage=storage),
+                 "cone512": ConeFDK(geoms["cone512"], storage=storage),
+                 "cone256_proj": ConeProjector(gc, storage=storage), "fan512_proj": FanProjector(gf, storage=storage,
+                                                                                                  step_factor=0.5)}
+        payloads = {k: [(s_ + torch.randn(s_.shape, device=dev, generator=torch.Generator(device=dev).manual_seed(
+            int(rng.integers(1 << 30)))) * (0.01 * float(s_.max()))).to(storage).cpu().pin_memory()]
+                    for k, s_ in clean.items()}
+        payloads["cone256_proj"] = [vol.to(storage).cpu().pin_memory()]
+        payloads["fan512_proj"] = [img.to(storage).cpu().pin_memory()]
+        ex = GpuExecutor(plans, slots=args.serve_slots, use_graphs=True,
+                         chunked={"cone512": args.serve_chunks} if args.serve_chunks > 1 else {})
+        ex.warm()
+        reqs = poisson_arrivals(9, args.serve_rate, args.serve_requests, mix)
+        for r in reqs:
+            r.payload = payloads[r.kind][0]
+        res = Scheduler(ex, slots=args.serve_slots, hbm_budget=160 << 30).run(
+            reqs, idle=lambda dt: time.sleep(dt) if dt > 0 else None)
+        rep = _rounded(latency_report(res))
+        log(f"serving with projections {name}: {rep['req_per_s']:.2f} req/s p50 {rep['p50_ms']:.1f} ms "
+            f"p99 {rep['p99_ms']:.1f} ms; " + ", ".join(f"{k} p50 {v['p50_ms']:.1f} p99 {v['p99_ms']:.1f}"
+                                                         for k, v in rep["per_kind"].items()))
+        out[name] = rep
         del ex, plans, payloads
         torch.cuda.synchronize()
         torch.cuda.empty_cache()

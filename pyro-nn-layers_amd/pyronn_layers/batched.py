@@ -93,17 +93,38 @@ def cone_projection3d_batched(volumes: Sequence[torch.Tensor], geometries: Seque
     return outs
 
 
+def _device_geometry(items, geometries, dev):
+    """Caller-held device arrays of each problem (objects with float32 device tensors P
+    [.., n, 3, 4], vo and vs [.., 3], step and mult [.., 1], and g, the geometry), e.g.
+    the server's FDK plans; checked against the geometries and the device."""
+    if len(items) != len(geometries):
+        raise ValueError(f"{len(items)} device geometries for {len(geometries)} problems")
+    for k, (it, g) in enumerate(zip(items, geometries)):
+        if it.g is not g:
+            raise ValueError(f"device geometry {k} belongs to another geometry")
+        for name, n in (("P", g.n_views * 12), ("vo", 3), ("vs", 3), ("step", 1), ("mult", 1)):
+            t = getattr(it, name)
+            if (not t.is_cuda or t.device != dev or t.dtype != torch.float32 or not t.is_contiguous()
+                    or t.numel() != n):
+                raise ValueError(f"device geometry {k}: {name} must be a contiguous float32 tensor of {n} "
+                                 f"values on {dev}")
+    return list(items)
+
+
 def cone_backprojection3d_batched(sinograms: Sequence[torch.Tensor], geometries: Sequence[geo.ConeGeometry],
                                   hardware_interp: bool = False, z_per_lane: int = 0,
-                                  blocks_per_cu: int = 0) -> List[torch.Tensor]:
-    """sinograms[k] [n_k, H_d,k, W_d,k] -> volumes [D_k, H_k, W_k] float32 (scaled by projection_multiplier)."""
+                                  blocks_per_cu: int = 0, device_geometry=None) -> List[torch.Tensor]:
+    """sinograms[k] [n_k, H_d,k, W_d,k] -> volumes [D_k, H_k, W_k] float32 (scaled by projection_multiplier).
+    device_geometry: optional per-problem device arrays already resident (P, vo, vs, step, mult;
+    e.g. FDK plans, whose mult is the FDK scale) used instead of uploading the geometries'."""
     _validate(sinograms, geometries, False)
     dev = sinograms[0].device
     dt = sinograms[0].dtype
     if any(s.dtype != dt for s in sinograms):
         raise TypeError("all sinograms of a batched launch share one storage dtype")
     sins = [s.contiguous() for s in sinograms]
-    geos = [_Geo(g, dev) for g in geometries]
+    geos = (_device_geometry(device_geometry, geometries, dev) if device_geometry is not None
+            else [_Geo(g, dev) for g in geometries])
     outs = [torch.empty(tuple(g.volume_shape), dtype=torch.float32, device=dev) for g in geometries]
     arr = _problems(sins, outs, geos)
     L = nat.lib()

@@ -37,6 +37,7 @@ import torch
 from . import geometry as geo
 from . import load as _load_ops
 from . import native as nat
+from .projectors import Capturable
 
 _DT = {torch.float32: nat.F32, torch.bfloat16: nat.BF16, torch.float16: nat.F16}
 
@@ -57,7 +58,7 @@ def _cuda_device(device) -> torch.device:
     return d
 
 
-class _Plan:
+class _Plan(Capturable):
     storage: torch.dtype
     device: torch.device
     # "fft": the fused LDS FFT filter (detector rows up to 4096 samples); "gemm": the
@@ -78,6 +79,17 @@ class _Plan:
         else:
             self.filter_impl = "gemm"
             self.ramp_matrix()
+
+    def footprint(self) -> int:
+        """Device bytes of one in-flight request: its input, the filtered copy, the GEMM
+        filter's workspace (two float32 copies) if that filter is used, the detector cells
+        of the cone backprojection (four values per pixel) and the float32 output."""
+        nin = int(np.prod(self.input_shape))
+        nout = int(np.prod(self.output_shape))
+        e = torch.tensor([], dtype=self.storage).element_size()
+        ws = 8 if self.filter_impl == "gemm" else 0
+        cells = 4 * e if isinstance(self, ConeFDK) else 0
+        return nin * (e + e + ws + cells) + nout * 4
 
     def ramp_matrix(self) -> torch.Tensor:
         """The Toeplitz ramp matrix of the GEMM filter (built on first use)."""
@@ -115,23 +127,6 @@ class _Plan:
                  _DT[self.storage], _p(ws), ws.numel(), _stream())
         return out
 
-    def capture(self, example: torch.Tensor, warmup: int = 2):
-        """Capture filter + backprojection for inputs shaped like `example` in a
-        HIP graph.  Returns (graph, static_input, static_output): copy a request
-        into static_input, replay the graph, read static_output."""
-        static_in = example.clone()
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            for _ in range(warmup):
-                self(static_in)
-        torch.cuda.current_stream().wait_stream(s)
-        torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            static_out = self(static_in)
-        return g, static_in, static_out
-
 
 class ConeFDK(_Plan):
     """FDK for a full circular cone-beam orbit described by a ConeGeometry."""
@@ -156,6 +151,8 @@ class ConeFDK(_Plan):
         nat.call("pyronn_fdk_cosine_weights", Hd, Wd, du, dv, (Wd - 1) / 2.0, (Hd - 1) / 2.0, g.sdd,
                  _p(self.weights), _stream())
         self._init_filter(Wd, du * g.sid / g.sdd)
+        self.input_shape = (1, n, Hd, Wd)
+        self.output_shape = (1,) + tuple(g.volume_shape)
 
     def filter(self, sino: torch.Tensor) -> torch.Tensor:
         B, n, Hd, Wd = sino.shape
@@ -279,6 +276,8 @@ class FanFBP(_Plan):
         post = ((g.sdd ** 2 + s ** 2) / g.sdd ** 2) * (g.sid / g.sdd) * (g.angular_range / (2 * math.pi))
         colw = torch.tensor(post, dtype=torch.float32, device=dev)
         self._init_filter(det, ds * g.sid / g.sdd, colw)
+        self.input_shape = (1, g.n_views, det)
+        self.output_shape = (1,) + tuple(g.volume_shape)
         torch.cuda.current_stream().synchronize()
 
     def filter(self, sino: torch.Tensor) -> torch.Tensor:
@@ -315,6 +314,8 @@ class ParallelFBP(_Plan):
         # The reference's pi / n scaling is the FBP weight for views over pi and,
         # since every direction then appears twice, also for views over 2 pi.
         self._init_filter(det, ds)
+        self.input_shape = (1, g.n_views, det)
+        self.output_shape = (1,) + tuple(g.volume_shape)
         torch.cuda.current_stream().synchronize()
 
     def filter(self, sino: torch.Tensor) -> torch.Tensor:

@@ -1,4 +1,5 @@
-"""Continuous-batching reconstruction server (FDK / FBP requests of mixed sizes).
+"""Continuous-batching server for reconstruction (FDK / FBP) and projection requests of
+mixed sizes.
 
 Requests arrive over time (e.g. a seeded Poisson process).  A request is
 admitted as soon as (a) it has arrived, (b) an execution slot is free and
@@ -34,7 +35,7 @@ class Request:
     rid: int
     kind: str                 # geometry class key
     arrival: float            # seconds after the start of the run
-    payload: Any = None       # executor-specific input (host sinogram)
+    payload: Any = None       # executor-specific input (host sinogram or volume)
 
 
 @dataclass
@@ -68,32 +69,46 @@ def poisson_arrivals(seed: int, rate_hz: float, count: int, kinds: Dict[str, flo
 class Scheduler:
     """Admission and completion bookkeeping; device work is delegated to an executor with
         footprint(kind) -> bytes, start(slot, request) -> handle,
-        poll(handle) -> bool, finish(handle) -> output."""
+        poll(handle) -> bool, finish(handle) -> output,
+    and, for grouping (group_max > 1), group_key(kind) -> key or None and
+    start_group(slot, requests) -> handle whose finish() returns one output per request.
 
-    def __init__(self, executor, slots: int, hbm_budget: int):
+    With group_max > 1, a request whose kind has a group key is started together with
+    the requests right behind it in the queue that have arrived, share its key and fit
+    the budget (at most group_max in all, never overtaking a request of another key):
+    one slot runs them as one batched launch."""
+
+    def __init__(self, executor, slots: int, hbm_budget: int, group_max: int = 1):
         if slots < 1:
             raise ValueError("need at least one slot")
+        if group_max < 1:
+            raise ValueError("group_max must be at least 1")
         self.ex = executor
         self.slots = slots
         self.budget = int(hbm_budget)
+        self.group_max = int(group_max)
 
     def run(self, requests: List[Request], clock: Callable[[], float] = time.perf_counter,
             idle: Callable[[float], None] = None, realtime: bool = True) -> List[Result]:
         pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
         free = list(range(self.slots))
         heapq.heapify(free)
-        active = {}            # slot -> (request, handle, result, footprint)
+        active = {}            # slot -> (results, handle, footprint, grouped)
         used = 0
         results: Dict[int, Result] = {}
         t0 = clock()
+        grouping = self.group_max > 1 and hasattr(self.ex, "group_key")
         while pending or active:
             now = clock() - t0
             # retire finished work, in slot order
             for slot in sorted(active):
-                req, h, res, fp = active[slot]
+                ress, h, fp, grouped = active[slot]
                 if self.ex.poll(h):
-                    res.output = self.ex.finish(h)
-                    res.done = clock() - t0
+                    out = self.ex.finish(h)
+                    done = clock() - t0
+                    for k, res in enumerate(ress):
+                        res.output = out[k] if grouped else out
+                        res.done = done
                     used -= fp
                     del active[slot]
                     heapq.heappush(free, slot)
@@ -110,17 +125,30 @@ class Scheduler:
                 if used + fp > self.budget:
                     break
                 pending.popleft()
+                group = [req]
+                key = self.ex.group_key(req.kind) if grouping else None
+                if key is not None:
+                    while (len(group) < self.group_max and pending and (pending[0].arrival <= now or not realtime)
+                           and self.ex.group_key(pending[0].kind) == key):
+                        f2 = self.ex.footprint(pending[0].kind)
+                        if used + fp + f2 > self.budget:
+                            break
+                        group.append(pending.popleft())
+                        fp += f2
                 slot = heapq.heappop(free)
-                res = Result(req.rid, req.kind, req.arrival, admitted=clock() - t0, slot=slot)
+                admitted = clock() - t0
+                ress = [Result(q.rid, q.kind, q.arrival, admitted=admitted, slot=slot) for q in group]
                 try:
-                    h = self.ex.start(slot, req)
+                    h = self.ex.start(slot, req) if len(group) == 1 else self.ex.start_group(slot, group)
                 except Exception as e:  # executor refused: report, free the slot
-                    res.error = str(e)
-                    results[req.rid] = res
+                    for res in ress:
+                        res.error = str(e)
+                        results[res.rid] = res
                     heapq.heappush(free, slot)
                     continue
-                results[req.rid] = res
-                active[slot] = (req, h, res, fp)
+                for res in ress:
+                    results[res.rid] = res
+                active[slot] = (ress, h, fp, len(group) > 1)
                 used += fp
             if not active and pending and realtime:
                 wait = pending[0].arrival - (clock() - t0)
@@ -149,9 +177,10 @@ def latency_report(results: List[Result]) -> Dict[str, Any]:
 
 
 class GpuExecutor:
-    """Runs FDK / FBP plans on one GPU.  Each slot has its own stream, device input
-    buffer, plan workspace, pinned host output buffer and (optionally) one captured
-    graph per geometry class.
+    """Runs plans on one GPU: FDK / FBP reconstructions (fdk.py) and forward projections
+    (projectors.py), one request kind per plan, its geometry resident on the device.
+    Each slot has its own stream, device input buffer, plan workspace, pinned host
+    output buffer and (optionally) one captured graph per request kind.
 
     Classes listed in `chunked` (cone FDK plans; value = number of view ranges) are run
     with ConeFDK.run_chunked instead: the slot's copy stream uploads view range k + 1 while
@@ -160,7 +189,7 @@ class GpuExecutor:
     a request's result is the same alone and in any mix)."""
 
     def __init__(self, plans: Dict[str, Any], slots: int, use_graphs: bool = True, keep_outputs: bool = False,
-                 device="cuda", chunked: Dict[str, int] | None = None):
+                 device="cuda", chunked: Dict[str, int] | None = None, group_kinds=()):
         import torch
 
         self.torch = torch
@@ -169,33 +198,32 @@ class GpuExecutor:
         self.use_graphs = use_graphs
         self.keep = keep_outputs
         self.chunked = dict(chunked or {})
+        # cone FDK kinds whose queued requests may share one batched backprojection launch
+        self.group_kinds = set(group_kinds) - set(self.chunked)
+        self._host_pool: Dict[tuple, list] = {}
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(slots)]
         self.copy_streams = [torch.cuda.Stream(device=self.device) for _ in range(slots)] if self.chunked else []
         self._slot: Dict[tuple, Dict[str, Any]] = {}
         self._fp = {k: self._estimate(k) for k in plans}
 
     def _in_shape(self, kind):
-        p = self.plans[kind]
-        g = p.g
-        if hasattr(g, "detector_shape") and isinstance(g.detector_shape, tuple):
-            return (1, g.n_views) + tuple(g.detector_shape)
-        return (1, g.n_views, g.detector_shape)
+        return tuple(self.plans[kind].input_shape)
 
     def _out_shape(self, kind):
-        return (1,) + tuple(self.plans[kind].g.volume_shape)
+        return tuple(self.plans[kind].output_shape)
 
     def _estimate(self, kind) -> int:
         p = self.plans[kind]
-        nin = int(np.prod(self._in_shape(kind)))
-        nout = int(np.prod(self._out_shape(kind)))
-        e = p.storage.itemsize
-        ws = 8 if getattr(p, "filter_impl", "gemm") == "gemm" else 0   # GEMM filter workspace: 2 x f32
         if kind in self.chunked:
             # two range buffers + filtered range + filter workspace + detector cells of one range + output
+            nin = int(np.prod(self._in_shape(kind)))
+            nout = int(np.prod(self._out_shape(kind)))
+            e = p.storage.itemsize
+            ws = 8 if getattr(p, "filter_impl", "gemm") == "gemm" else 0   # GEMM filter workspace: 2 x f32
             nk = -(-nin // self.chunked[kind])
             return nk * (2 * e + e + ws + 4 * e) + nout * 4
-        # input + filtered + filter workspace + detector cells (4 x) + output
-        return nin * (e + e + ws + 4 * e) + nout * 4
+        # the plan's own count: request buffers, workspaces and output
+        return int(p.footprint())
 
     def footprint(self, kind) -> int:
         return self._fp[kind]
@@ -266,9 +294,50 @@ class GpuExecutor:
             ev.record(st)
         return (ev, b)
 
+    def group_key(self, kind):
+        """Requests of group kinds with the same storage dtype can share a launch."""
+        if kind not in self.group_kinds:
+            return None
+        return ("cone_fdk", self.plans[kind].storage)
+
+    def start_group(self, slot, reqs):
+        """Upload and filter each request, then backproject all of them in ONE batched
+        launch (pyronn_cone_backprojection3d_batched over their geometries, each plan's
+        device-resident matrices and FDK scale).  The batched launch gives every volume
+        bit for bit what the plan's own backprojection gives, so a request's result does
+        not depend on its group."""
+        from . import batched
+
+        torch = self.torch
+        st = self.streams[slot]
+        seen: Dict[str, int] = {}
+        hosts = []
+        with torch.cuda.stream(st):
+            qs, plans = [], []
+            for r in reqs:
+                p = self.plans[r.kind]
+                x = torch.empty(p.input_shape, dtype=p.storage, device=self.device)
+                x.copy_(r.payload, non_blocking=True)
+                qs.append(p.filter(x)[0])
+                plans.append(p)
+            vols = batched.cone_backprojection3d_batched(qs, [p.g for p in plans], device_geometry=plans)
+            for r, v in zip(reqs, vols):
+                j = seen.get(r.kind, 0)
+                seen[r.kind] = j + 1
+                pool = self._host_pool.setdefault((slot, r.kind), [])
+                while len(pool) <= j:
+                    pool.append(torch.empty(self._out_shape(r.kind), dtype=torch.float32, pin_memory=True))
+                pool[j][0].copy_(v, non_blocking=True)
+                hosts.append(pool[j])
+            ev = torch.cuda.Event()
+            ev.record(st)
+        return (ev, {"group": hosts})
+
     def poll(self, h) -> bool:
         return h[0].query()
 
     def finish(self, h):
         h[0].synchronize()
+        if "group" in h[1]:
+            return [x.clone() if self.keep else None for x in h[1]["group"]]
         return h[1]["host_out"].clone() if self.keep else None

@@ -92,3 +92,55 @@ def test_executor_refusal_frees_the_slot():
     res = Scheduler(ex, slots=1, hbm_budget=10).run([Request(0, "a", 0.0), Request(1, "a", 0.0)], clock=clk,
                                                     idle=clk.sleep)
     assert res[0].error == "no memory" and res[1].error is None and res[1].output == 10
+
+
+class GroupingExecutor(FakeExecutor):
+    """Kinds 'g1'/'g2' share a group key; a group takes the longest member's time."""
+
+    def __init__(self, clock, dur, fp):
+        super().__init__(clock, dur, fp)
+        self.groups = []
+
+    def group_key(self, kind):
+        return "g" if kind.startswith("g") else None
+
+    def start_group(self, slot, reqs):
+        self.groups.append([r.rid for r in reqs])
+        self.running += 1
+        self.max_running = max(self.max_running, self.running)
+        return {"end": self.clock() + max(self.dur[r.kind] for r in reqs), "rids": [r.rid for r in reqs]}
+
+    def finish(self, h):
+        self.running -= 1
+        if "rids" in h:
+            return [rid * 10 for rid in h["rids"]]
+        return h["rid"] * 10
+
+
+def test_grouping_takes_a_contiguous_run_of_arrived_requests():
+    clk = FakeClock()
+    ex = GroupingExecutor(clk, {"g1": 1.0, "g2": 0.5, "x": 0.2}, {"g1": 10, "g2": 10, "x": 10})
+    reqs = [Request(0, "g1", 0.0), Request(1, "g2", 0.0), Request(2, "g1", 0.0), Request(3, "x", 0.0),
+            Request(4, "g1", 0.0), Request(5, "g2", 5.0)]
+    res = Scheduler(ex, slots=2, hbm_budget=1000, group_max=3).run(reqs, clock=clk, idle=clk.sleep)
+    assert all(r.error is None for r in res)
+    assert [r.output for r in res] == [i * 10 for i in range(6)]
+    # 0, 1, 2 start together (the run stops at the 'x' request, no overtaking); 4 runs alone
+    # behind 'x'; 5 has not arrived when 4 starts
+    assert ex.groups[0] == [0, 1, 2]
+    assert [0, 1, 2] == [r.rid for r in res if r.slot == res[0].slot and r.admitted == res[0].admitted]
+    assert res[4].admitted >= res[3].admitted
+    assert res[5].admitted >= 5.0
+
+
+def test_grouping_respects_the_budget_and_group_max():
+    clk = FakeClock()
+    ex = GroupingExecutor(clk, {"g1": 1.0}, {"g1": 40})
+    reqs = [Request(i, "g1", 0.0) for i in range(5)]
+    Scheduler(ex, slots=1, hbm_budget=100, group_max=4).run(reqs, clock=clk, idle=clk.sleep)
+    assert ex.groups[0] == [0, 1]             # a third would exceed the 100-byte budget
+    assert all(len(g) <= 2 for g in ex.groups)
+    ex2 = GroupingExecutor(clk, {"g1": 1.0}, {"g1": 1})
+    Scheduler(ex2, slots=1, hbm_budget=100, group_max=2).run([Request(i, "g1", 0.0) for i in range(5)], clock=clk,
+                                                             idle=clk.sleep)
+    assert [len(g) for g in ex2.groups] == [2, 2]   # the last request runs alone through start()
