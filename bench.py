@@ -732,8 +732,8 @@ def batched_bench(args):
         vin = [v.to(storage) for v in vols]
         log(f"uneven batch {name}: tuning")
         tf = tuner.forward(vin, geoms)
-        sinos, t_fwd = timed(lambda: batched.cone_projection3d_batched(vin, geoms,
-                                                                       blocks_per_cu=tf["blocks_per_cu"]))
+        sinos, t_fwd = timed(lambda: batched.cone_projection3d_batched(vin, geoms, blocks_per_cu=tf["blocks_per_cu"],
+                                                                       forward_tile=tf["forward_tile"]))
         sin = [s_.to(storage) for s_ in sinos]
         tb = tuner.backward(sin, geoms)
         _, t_bp = timed(lambda: batched.cone_backprojection3d_batched(sin, geoms, z_per_lane=tb["z_per_lane"],
@@ -741,7 +741,8 @@ def batched_bench(args):
         # the same problems launched one after another through the single-problem ops
         def seq_fwd():
             for v, g in zip(vin, geoms):
-                batched.cone_projection3d_batched([v], [g], blocks_per_cu=tf["blocks_per_cu"])
+                batched.cone_projection3d_batched([v], [g], blocks_per_cu=tf["blocks_per_cu"],
+                                                  forward_tile=tf["forward_tile"])
         def seq_bp():
             for s_, g in zip(sin, geoms):
                 batched.cone_backprojection3d_batched([s_], [g], z_per_lane=tb["z_per_lane"])
@@ -759,7 +760,9 @@ def batched_bench(args):
             f"{ts_b * 1e3:.1f} ms), rel L2 vs FP64 {err:.2e}")
         out[name] = {"gsamples_s": round((S + U) / (t_fwd + t_bp) / 1e9, 2), "t_fwd_ms": round(t_fwd * 1e3, 2),
                      "t_bp_ms": round(t_bp * 1e3, 2), "sequential_t_fwd_ms": round(ts_f * 1e3, 2),
-                     "sequential_t_bp_ms": round(ts_b * 1e3, 2), "tuned_fwd": tf["blocks_per_cu"],
+                     "sequential_t_bp_ms": round(ts_b * 1e3, 2),
+                     "tuned_fwd": {"blocks_per_cu": tf["blocks_per_cu"], "tile": tf["tile_shape"],
+                                   "trials_ms": {k: round(v, 2) for k, v in tf["trials_ms"].items()}},
                      "tuned_bp": [tb["z_per_lane"], tb["blocks_per_cu"]],
                      "rel_l2_vs_fp64_fwd_views_0_1_of_128^3": err}
         del sinos, sin, vin

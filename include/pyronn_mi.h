@@ -244,10 +244,14 @@ typedef struct pyronn_cone_problem {
 
 size_t pyronn_cone_projection3d_batched_workspace_size(const pyronn_cone_problem *problems, int count,
                                                        int dtype);
-/* blocks_per_cu: persistent workgroups per CU (<= 0: 8), a tuning parameter. */
+/* Tuning parameters (results are the same bits for every value):
+ * blocks_per_cu: persistent workgroups per CU (<= 0: 8);
+ * forward_tile: the detector tile of a 256-lane workgroup and the waves within it,
+ *   0: 16 x 16 pixels, waves of 4 columns x 16 rows (the single-problem shape),
+ *   1: 8 x 32, waves 2 x 32;  2: 32 x 8, waves 8 x 8;  3: 16 x 16, waves 8 x 8. */
 int pyronn_cone_projection3d_batched(const pyronn_cone_problem *problems, int count, int dtype,
-                                     int hardware_interp, int blocks_per_cu, void *workspace,
-                                     size_t workspace_bytes, pyronn_stream_t stream);
+                                     int hardware_interp, int blocks_per_cu, int forward_tile,
+                                     void *workspace, size_t workspace_bytes, pyronn_stream_t stream);
 
 size_t pyronn_cone_backprojection3d_batched_workspace_size(const pyronn_cone_problem *problems,
                                                            int count, int dtype);

@@ -406,8 +406,21 @@ struct FwdProblem {
   long long begin;       // first work item of the problem in a batched launch (multiple of 8)
 };
 
+// Forward block shapes: a 256-lane block covers TC detector columns x TR rows and each of
+// its four waves WC columns x WR = 64 / WC rows (waves side by side along the columns,
+// then along the rows), lanes numbered rows fastest within a wave.  FwdTile16 is the
+// shape of the single-problem launches; the batched launch takes any of kFwdTiles.
+template <int TC_, int TR_, int WC_> struct FwdTile {
+  static constexpr int TC = TC_, TR = TR_, WC = WC_, WR = 64 / WC_, WX = TC_ / WC_;
+  static_assert(TC * TR == 256 && WC * WR == 64 && (TC / WC) * (TR / WR) == 4, "four 64-lane waves");
+};
+using FwdTile16 = FwdTile<16, 16, 4>;
+constexpr int kFwdTiles = 4;  // 0: 16 x 16 (4 x 16 waves), 1: 8 x 32 (2 x 32), 2: 32 x 8 (8 x 8), 3: 16 x 16 (8 x 8)
+__host__ __device__ constexpr int fwd_tile_cols(int t) { return t == 1 ? 8 : t == 2 ? 32 : 16; }
+__host__ __device__ constexpr int fwd_tile_rows(int t) { return t == 1 ? 32 : t == 2 ? 8 : 16; }
+
 // DIRECT: P.cells is the volume [D][H][W] itself, sampled with sample_direct.
-template <typename TS, bool HW, typename IT, bool DIRECT = false>
+template <typename TS, bool HW, typename IT, bool DIRECT = false, typename TL = FwdTile16>
 __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int xcd = (int)(q % kXcds);
@@ -419,8 +432,8 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   const long long w_ = qq - (long long)band * per_row;
   const int tx = (int)(w_ / P.n);
   const int view = (int)(w_ - (long long)tx * P.n);
-  const int col = tx * 16 + wave * 4 + (lane >> 4);
-  const int row = ty * 16 + (lane & 15);
+  const int col = tx * TL::TC + (wave % TL::WX) * TL::WC + lane / TL::WR;
+  const int row = ty * TL::TR + (wave / TL::WX) * TL::WR + lane % TL::WR;
   const int W = P.W, H = P.H, D = P.D, Hd = P.Hd, Wd = P.Wd;
   if (col >= Wd || row >= Hd) return;
 
@@ -543,7 +556,8 @@ __device__ __forceinline__ long long claim(unsigned long long *counters, int xcd
 // the single-launch order of each problem (problems concatenated, each a multiple of
 // 8 items); XCD x processes items x, x + 8, x + 16, ... in claim order, so the blocks
 // resident on an XCD always work on consecutive items of its detector row bands.
-template <typename TS, bool HW, typename IT>
+// TL is the block shape (the problems' tiles_x / tiles_y are counted in it).
+template <typename TS, bool HW, typename IT, typename TL>
 __global__ void __launch_bounds__(256)
 cone_fwd_persistent_kernel(FwdBatch B, unsigned long long *counters) {
   // one claim takes kFwdClaim consecutive items of the XCD's sequence (a 16 x 16 tile of
@@ -559,7 +573,7 @@ cone_fwd_persistent_kernel(FwdBatch B, unsigned long long *counters) {
       const long long w = (t0 + j) * kXcds + xcd;
       if (w >= B.total) break;
       while (k + 1 < B.count && w >= B.p[k + 1].begin) ++k;
-      fwd_block<TS, HW, IT>(B.p[k], w - B.p[k].begin);
+      fwd_block<TS, HW, IT, false, TL>(B.p[k], w - B.p[k].begin);
     }
   }
 }
@@ -853,9 +867,11 @@ static bool det_fidx(int Hd, int Wd) { return (long long)(Hd + 2) * (Wd + 2) <= 
 constexpr long long kMaxGridX = (1LL << 24) - 1;
 constexpr int kMaxGridYZ = 65535;
 
-// Blocks of one forward launch per batch item: 8 XCD lanes x tile-row bands x views x tiles.
-static long long fwd_blocks(int n, int Hd, int Wd) {
-  const long long tiles_x = (Wd + 15) / 16, tiles_y = (Hd + 15) / 16;
+// Blocks of one forward launch per batch item: 8 XCD lanes x tile-row bands x views x tiles
+// (tile shape `tile`, kFwdTiles).
+static long long fwd_blocks(int n, int Hd, int Wd, int tile = 0) {
+  const int tc = fwd_tile_cols(tile), tr = fwd_tile_rows(tile);
+  const long long tiles_x = (Wd + tc - 1) / tc, tiles_y = (Hd + tr - 1) / tr;
   return (long long)kXcds * ((tiles_y + kXcds - 1) / kXcds) * n * tiles_x;
 }
 // Views per forward launch (0: one view alone exceeds a launch).
@@ -1324,14 +1340,26 @@ static size_t fwd_problem_bytes(const pyronn_cone_problem &p, int dtype) {
 
 constexpr size_t kCounterBytes = 256;  // per-XCD work counters at the start of a batched workspace
 
+template <typename TS, bool HW, typename IT>
+static void launch_fwd_persistent(const FwdBatch &fb, int tile, dim3 g, unsigned long long *counters,
+                                  hipStream_t s) {
+  switch (tile) {
+    case 1: cone_fwd_persistent_kernel<TS, HW, IT, FwdTile<8, 32, 2>><<<g, dim3(256), 0, s>>>(fb, counters); break;
+    case 2: cone_fwd_persistent_kernel<TS, HW, IT, FwdTile<32, 8, 8>><<<g, dim3(256), 0, s>>>(fb, counters); break;
+    case 3: cone_fwd_persistent_kernel<TS, HW, IT, FwdTile<16, 16, 8>><<<g, dim3(256), 0, s>>>(fb, counters); break;
+    default: cone_fwd_persistent_kernel<TS, HW, IT, FwdTile16><<<g, dim3(256), 0, s>>>(fb, counters); break;
+  }
+}
+
 template <typename TS>
-static int fwd_batched(const pyronn_cone_problem *pr, int count, bool hw, int bpc, char *ws,
+static int fwd_batched(const pyronn_cone_problem *pr, int count, bool hw, int bpc, int tile, char *ws,
                        hipStream_t s) {
   unsigned long long *counters = (unsigned long long *)ws;
   ws += kCounterBytes;
   FwdBatch fb{};
   long long begin = 0;
   bool small = true;
+  const int tc = fwd_tile_cols(tile), tr = fwd_tile_rows(tile);
   for (int k = 0; k < count; ++k) {
     const pyronn_cone_problem &p = pr[k];
     float *inv_ar = (float *)ws;
@@ -1344,8 +1372,8 @@ static int fwd_batched(const pyronn_cone_problem *pr, int count, bool hw, int bp
     small = small && fwd_small_index(p.D, p.H, p.W);
     fb.p[k] = FwdProblem{cells, p.output, inv_ar, src, p.step_size, p.volume_spacing,
                          p.W, p.H, p.D, p.n_views, p.det_h, p.det_w,
-                         (p.det_w + 15) / 16, (p.det_h + 15) / 16, begin};
-    begin += fwd_blocks(p.n_views, p.det_h, p.det_w);
+                         (p.det_w + tc - 1) / tc, (p.det_h + tr - 1) / tr, begin};
+    begin += fwd_blocks(p.n_views, p.det_h, p.det_w, tile);
   }
   fb.count = count;
   fb.total = begin;
@@ -1355,11 +1383,11 @@ static int fwd_batched(const pyronn_cone_problem *pr, int count, bool hw, int bp
   reset_counters_kernel<<<1, 64, 0, s>>>(counters);
   const dim3 g((unsigned)grid);
   if (small) {
-    if (hw) cone_fwd_persistent_kernel<TS, true, int><<<g, dim3(256), 0, s>>>(fb, counters);
-    else cone_fwd_persistent_kernel<TS, false, int><<<g, dim3(256), 0, s>>>(fb, counters);
+    if (hw) launch_fwd_persistent<TS, true, int>(fb, tile, g, counters, s);
+    else launch_fwd_persistent<TS, false, int>(fb, tile, g, counters, s);
   } else {
-    if (hw) cone_fwd_persistent_kernel<TS, true, long long><<<g, dim3(256), 0, s>>>(fb, counters);
-    else cone_fwd_persistent_kernel<TS, false, long long><<<g, dim3(256), 0, s>>>(fb, counters);
+    if (hw) launch_fwd_persistent<TS, true, long long>(fb, tile, g, counters, s);
+    else launch_fwd_persistent<TS, false, long long>(fb, tile, g, counters, s);
   }
   return PYRONN_OK;
 }
@@ -1424,11 +1452,13 @@ extern "C" size_t pyronn_cone_projection3d_batched_workspace_size(const pyronn_c
 
 extern "C" int pyronn_cone_projection3d_batched(const pyronn_cone_problem *problems, int count,
                                                 int dtype, int hardware_interp, int blocks_per_cu,
-                                                void *workspace, size_t workspace_bytes,
+                                                int forward_tile, void *workspace, size_t workspace_bytes,
                                                 pyronn_stream_t stream) {
   clear_error();
   int rc = check_problems("cone_projection3d_batched", problems, count, dtype, true);
   if (rc != PYRONN_OK) return rc;
+  PYRONN_CHECK_ARG(forward_tile >= 0 && forward_tile < kFwdTiles,
+                   "cone_projection3d_batched: forward_tile %d not in [0, %d)", forward_tile, kFwdTiles);
   PYRONN_CHECK_ARG(workspace != nullptr, "cone_projection3d_batched: null workspace");
   const size_t need = pyronn_cone_projection3d_batched_workspace_size(problems, count, dtype);
   if (workspace_bytes < need) {
@@ -1439,9 +1469,9 @@ extern "C" int pyronn_cone_projection3d_batched(const pyronn_cone_problem *probl
   hipStream_t s = (hipStream_t)stream;
   const bool hw = hardware_interp != 0;
   char *ws = (char *)workspace;
-  if (dtype == PYRONN_F32) rc = fwd_batched<float>(problems, count, hw, blocks_per_cu, ws, s);
-  else if (dtype == PYRONN_BF16) rc = fwd_batched<bf16s>(problems, count, hw, blocks_per_cu, ws, s);
-  else rc = fwd_batched<_Float16>(problems, count, hw, blocks_per_cu, ws, s);
+  if (dtype == PYRONN_F32) rc = fwd_batched<float>(problems, count, hw, blocks_per_cu, forward_tile, ws, s);
+  else if (dtype == PYRONN_BF16) rc = fwd_batched<bf16s>(problems, count, hw, blocks_per_cu, forward_tile, ws, s);
+  else rc = fwd_batched<_Float16>(problems, count, hw, blocks_per_cu, forward_tile, ws, s);
   PYRONN_CHECK_LAUNCH("cone_fwd_persistent_kernel");
   return rc;
 }

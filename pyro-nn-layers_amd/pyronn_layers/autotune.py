@@ -1,13 +1,16 @@
 """Project-written autotuner for the batched persistent launches.
 
 Tunable parameters (all preserve results bit for bit):
-  forward          blocks_per_cu in {2, 4, 8, 16}  (persistent grid = CUs x value)
+  forward          forward_tile in {0, 1, 2, 3}: the detector tile of a workgroup and its
+                   waves (16x16 of 4x16 waves, 8x32 of 2x32, 32x8 of 8x8, 16x16 of 8x8),
+                   timed at 8 workgroups per CU; then blocks_per_cu in {4, 16} for the
+                   best tile (persistent grid = CUs x value)
   backprojection   z_per_lane in {4, 8} x blocks_per_cu in {0, 4, 8, 16}
                    (0: one workgroup per item; otherwise a persistent grid)
 Each candidate is timed with HIP events (best of `reps` after one warm-up) and
 the winner is cached in a JSON file keyed by the batch signature (shapes,
-views, detector sizes, storage dtype, device name), so a later call with the
-same signature reuses it without timing.
+views, detector sizes, storage dtype, interpolation mode, device name), so a later
+call with the same signature reuses it without timing.
 """
 from __future__ import annotations
 
@@ -23,10 +26,14 @@ from . import geometry as geo
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "pyronn_mi_autotune.json")
 
 
-def signature(kind: str, tensors: Sequence[torch.Tensor], geometries: Sequence[geo.ConeGeometry]) -> str:
+TILE_SHAPES = {0: "16x16/4x16", 1: "8x32/2x32", 2: "32x8/8x8", 3: "16x16/8x8"}   # tile / wave (cols x rows)
+
+
+def signature(kind: str, tensors: Sequence[torch.Tensor], geometries: Sequence[geo.ConeGeometry],
+              hardware_interp: bool = False) -> str:
     dev = torch.cuda.get_device_name(tensors[0].device)
     parts = [f"{tuple(g.volume_shape)}:{g.n_views}:{tuple(g.detector_shape)}" for g in geometries]
-    return f"{kind}|{tensors[0].dtype}|{dev}|" + ";".join(parts)
+    return f"{kind}|{tensors[0].dtype}|hw={int(bool(hardware_interp))}|{dev}|" + ";".join(parts)
 
 
 def _time(fn, reps: int) -> float:
@@ -65,20 +72,28 @@ class Autotuner:
         os.replace(tmp, self.path)
 
     def forward(self, volumes, geometries, hardware_interp=False) -> dict:
-        key = signature("fwd", volumes, geometries)
+        key = signature("fwd", volumes, geometries, hardware_interp)
         if key not in self.cache:
             trials = {}
-            for bpc in (2, 4, 8, 16):
-                trials[bpc] = _time(lambda: batched.cone_projection3d_batched(volumes, geometries, hardware_interp,
-                                                                              blocks_per_cu=bpc), self.reps)
+
+            def trial(bpc, tile):
+                trials[(bpc, tile)] = _time(lambda: batched.cone_projection3d_batched(
+                    volumes, geometries, hardware_interp, blocks_per_cu=bpc, forward_tile=tile), self.reps)
+
+            for tile in TILE_SHAPES:
+                trial(8, tile)
+            tile = min(trials, key=trials.get)[1]
+            for bpc in (4, 16):
+                trial(bpc, tile)
             best = min(trials, key=trials.get)
-            self.cache[key] = {"blocks_per_cu": best, "ms": trials[best],
-                               "trials_ms": {str(k): v for k, v in trials.items()}}
+            self.cache[key] = {"blocks_per_cu": best[0], "forward_tile": best[1], "tile_shape": TILE_SHAPES[best[1]],
+                               "ms": trials[best],
+                               "trials_ms": {f"{k[0]}/{TILE_SHAPES[k[1]]}": v for k, v in trials.items()}}
             self._save()
         return self.cache[key]
 
     def backward(self, sinograms, geometries, hardware_interp=False) -> dict:
-        key = signature("bp", sinograms, geometries)
+        key = signature("bp", sinograms, geometries, hardware_interp)
         if key not in self.cache:
             trials = {}
             for zpl in (4, 8):

@@ -73,8 +73,11 @@ def _check(n):
 
 
 def cone_projection3d_batched(volumes: Sequence[torch.Tensor], geometries: Sequence[geo.ConeGeometry],
-                              hardware_interp: bool = False, blocks_per_cu: int = 0) -> List[torch.Tensor]:
-    """volumes[k] [D_k, H_k, W_k] (one storage dtype for all) -> sinograms [n_k, H_d,k, W_d,k] float32."""
+                              hardware_interp: bool = False, blocks_per_cu: int = 0,
+                              forward_tile: int = 0) -> List[torch.Tensor]:
+    """volumes[k] [D_k, H_k, W_k] (one storage dtype for all) -> sinograms [n_k, H_d,k, W_d,k] float32.
+    blocks_per_cu and forward_tile (0..3: 16x16, 8x32, 32x8 and 16x16 with 8x8 waves) are launch
+    shapes that never change a result."""
     _validate(volumes, geometries, True)
     dev = volumes[0].device
     dt = volumes[0].dtype
@@ -88,7 +91,7 @@ def cone_projection3d_batched(volumes: Sequence[torch.Tensor], geometries: Seque
     ws_bytes = L.pyronn_cone_projection3d_batched_workspace_size(arr, len(vols), _DT[dt])
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     nat.call("pyronn_cone_projection3d_batched", arr, len(vols), _DT[dt], int(bool(hardware_interp)),
-             int(blocks_per_cu), ctypes.c_void_p(ws.data_ptr()), ws_bytes,
+             int(blocks_per_cu), int(forward_tile), ctypes.c_void_p(ws.data_ptr()), ws_bytes,
              ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
     return outs
 

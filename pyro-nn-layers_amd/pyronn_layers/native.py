@@ -15,6 +15,7 @@ HIP_LIB = os.path.join(_HERE, "lib", "libpyronn_mi.so")
 PYRONN_OK = 0
 F32, BF16, F16 = 0, 1, 2
 MAX_BATCHED_PROBLEMS = 16
+FORWARD_TILES = 4   # forward_tile values of pyronn_cone_projection3d_batched
 
 
 class ConeProblem(ctypes.Structure):
@@ -55,7 +56,7 @@ _SIGS = {
     "pyronn_fan_backprojection2d": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i,
                                          _vp, _vp]),
     "pyronn_cone_projection3d_batched_workspace_size": (_sz, [_vp, _i, _i]),
-    "pyronn_cone_projection3d_batched": (_i, [_vp, _i, _i, _i, _i, _vp, _sz, _vp]),
+    "pyronn_cone_projection3d_batched": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _sz, _vp]),
     "pyronn_cone_backprojection3d_batched_workspace_size": (_sz, [_vp, _i, _i]),
     "pyronn_cone_backprojection3d_batched": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _sz, _vp]),
     "pyronn_fdk_ramp_matrix": (_i, [_i, ctypes.c_double, _vp, _vp, _vp]),

@@ -187,17 +187,21 @@ def test_batched_argument_validation():
     P = nat.ConeProblem
     good = P(0x1000, 0x2000, 0x3000, 0x4000, 0x5000, 0x6000, 0x7000, 4, 4, 4, 2, 4, 4)
     arr = (P * 17)(*([good] * 17))
-    assert L.pyronn_cone_projection3d_batched(arr, 0, 0, 0, 0, _fake(), 1 << 30, None) == 1
+    assert L.pyronn_cone_projection3d_batched(arr, 0, 0, 0, 0, 0, _fake(), 1 << 30, None) == 1
     assert "count" in nat.last_error()
-    assert L.pyronn_cone_projection3d_batched(arr, 17, 0, 0, 0, _fake(), 1 << 30, None) == 1
+    assert L.pyronn_cone_projection3d_batched(arr, 17, 0, 0, 0, 0, _fake(), 1 << 30, None) == 1
     bad = (P * 2)(good, P(0, 0x2000, 0x3000, 0x4000, 0x5000, 0x6000, 0x7000, 4, 4, 4, 2, 4, 4))
-    assert L.pyronn_cone_projection3d_batched(bad, 2, 0, 0, 0, _fake(), 1 << 30, None) == 1
+    assert L.pyronn_cone_projection3d_batched(bad, 2, 0, 0, 0, 0, _fake(), 1 << 30, None) == 1
     assert "problem 1" in nat.last_error()
     assert L.pyronn_cone_backprojection3d_batched(arr, 2, 0, 0, 5, 0, _fake(), 1 << 30, None) == 1
     assert "z_per_lane" in nat.last_error()
     need = L.pyronn_cone_projection3d_batched_workspace_size(arr, 2, 0)
     assert need == 256 + 2 * L.pyronn_cone_projection3d_workspace_size(1, 4, 4, 4, 2, 0)
-    assert L.pyronn_cone_projection3d_batched(arr, 2, 0, 0, 0, _fake(), need - 1, None) == 2
+    assert L.pyronn_cone_projection3d_batched(arr, 2, 0, 0, 0, 0, _fake(), need - 1, None) == 2
+    # forward_tile outside 0..3
+    assert L.pyronn_cone_projection3d_batched(arr, 2, 0, 0, 0, 4, _fake(), need, None) == 1
+    assert "forward_tile" in nat.last_error()
+    assert L.pyronn_cone_projection3d_batched(arr, 2, 0, 0, 0, -1, _fake(), need, None) == 1
 
 
 def test_fft_filter_limits_and_validation():

@@ -55,10 +55,11 @@ def test_batched_equals_single(storage, hw):
     pl.load()
     geoms, vols = _problems()
     vt = [torch.tensor(v, device="cuda").to(storage) for v in vols]
-    for bpc in (0, 2):
-        sinos = batched.cone_projection3d_batched(vt, geoms, hw, blocks_per_cu=bpc)
-        for s, v, g in zip(sinos, vt, geoms):
-            assert torch.equal(s, _single_fwd(v, g, hw))
+    singles = [_single_fwd(v, g, hw) for v, g in zip(vt, geoms)]
+    for bpc, tile in ((0, 0), (2, 0), (4, 1), (8, 2), (16, 3)):   # every forward tile shape
+        sinos = batched.cone_projection3d_batched(vt, geoms, hw, blocks_per_cu=bpc, forward_tile=tile)
+        for s, s1 in zip(sinos, singles):
+            assert torch.equal(s, s1), f"blocks_per_cu {bpc} tile {tile}"
     st = [s.to(storage) for s in sinos]
     for zpl, bpc in ((0, 0), (4, -1), (4, 2), (8, 16)):   # one workgroup per item, then persistent grids
         recs = batched.cone_backprojection3d_batched(st, geoms, hw, z_per_lane=zpl, blocks_per_cu=bpc)
@@ -74,9 +75,12 @@ def test_autotuner_caches_choice(tmp_path):
     vt = [torch.tensor(v, device="cuda") for v in vols]
     at = Autotuner(str(tmp_path / "tune.json"), reps=1)
     f = at.forward(vt, geoms)
-    assert f["blocks_per_cu"] in (2, 4, 8, 16)
+    assert f["blocks_per_cu"] in (4, 8, 16)
+    assert f["forward_tile"] in (0, 1, 2, 3) and "x" in f["tile_shape"]
+    assert len(f["trials_ms"]) == 6                      # four tiles, then two more grid sizes
     from pyronn_layers import batched
-    sinos = batched.cone_projection3d_batched(vt, geoms, blocks_per_cu=f["blocks_per_cu"])
+    sinos = batched.cone_projection3d_batched(vt, geoms, blocks_per_cu=f["blocks_per_cu"],
+                                              forward_tile=f["forward_tile"])
     b = at.backward(sinos, geoms)
     assert b["z_per_lane"] in (4, 8)
     again = Autotuner(str(tmp_path / "tune.json"))
