@@ -328,6 +328,122 @@ __device__ __forceinline__ float cone_sample(const TS *__restrict__ c, IT Dc, IT
     return sample_cells<TS, CLAMP, IT>(c, Dc, HDc, px, py, pz, fW, fH, fD, r);
 }
 
+// The 16-bit forward's march, streamed: every sample loads its cell, and the load of the
+// next sample is issued before the current sample is blended, so the blend's arithmetic
+// covers the load's latency.  (The 16-bit forward issues one load per march iteration and
+// is bound by latency and VALU issue rather than by the address path; the f32 forward,
+// which issues two and is bound by the address path, keeps march_ray.)  Same samples, same
+// positions, same blend on the same values, accumulated in the same order as march_ray:
+// the tail's re-evaluation of the last sample is that sample's value, reused.
+template <typename TS, bool HW, bool CLAMP, typename IT>
+__device__ __forceinline__ void locate16(float px, float py, float pz, IT Dc, IT HDc, float fW, float fH, float fD,
+                                         typename CellKey<IT>::type &cell, float &xd, float &yd, float &zd) {
+  if (HW) {  // tex3D(p + 0.5): the sampler subtracts the half texel again
+    px = (px + 0.5f) - 0.5f;
+    py = (py + 0.5f) - 0.5f;
+    pz = (pz + 0.5f) - 0.5f;
+  }
+  if (CLAMP) {
+    px = clamp_apron(px, fW);
+    py = clamp_apron(py, fH);
+    pz = clamp_apron(pz, fD);
+  }
+  const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+  xd = px - fx;
+  yd = py - fy;
+  zd = pz - fz;
+  if constexpr (sizeof(IT) == 4) {
+    const int yz = (int)fmaf(fy, (float)Dc, fz);
+    cell = (unsigned)(__mul24((int)fx, HDc) + yz + (HDc + Dc + 1));
+  } else {
+    cell = (unsigned long long)((long long)(int)fx * HDc + (long long)(int)fy * Dc + (long long)(int)fz +
+                                (HDc + Dc + 1));
+  }
+}
+
+template <typename TS, typename K>
+__device__ __forceinline__ u32x4_a8 load16(const TS *__restrict__ cells, K cell) {
+  return *reinterpret_cast<const u32x4_a8 *>(reinterpret_cast<const uint2 *>(cells) + cell);
+}
+
+// The trilinear blend of sample_cells on a loaded 16-bit cell (quads at z and z + 1).
+template <typename TS>
+__device__ __forceinline__ float blend16(const u32x4_a8 q, float xd, float yd, float zd) {
+  float c[8];
+  if constexpr (std::is_same<TS, _Float16>::value) {
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      _Float16 h[2];
+      __builtin_memcpy(h, &w[i >> 1], 4);
+      c[i] = (float)h[i & 1];
+    }
+  } else {
+    unpack2(q.x, TS{}, c[0], c[1]);
+    unpack2(q.y, TS{}, c[2], c[3]);
+    unpack2(q.z, TS{}, c[4], c[5]);
+    unpack2(q.w, TS{}, c[6], c[7]);
+  }
+  const float p00 = lerp_ref(c[0], c[4], zd);
+  const float p01 = lerp_ref(c[1], c[5], zd);
+  const float p10 = lerp_ref(c[2], c[6], zd);
+  const float p11 = lerp_ref(c[3], c[7], zd);
+  const float p0 = lerp_ref(p00, p10, yd);
+  const float p1 = lerp_ref(p01, p11, yd);
+  return lerp_ref(p0, p1, xd);
+}
+
+template <typename TS, bool HW, bool GUARDED, typename IT>
+__device__ __forceinline__ float march_ray_stream(const TS *__restrict__ v, IT Dc, IT HDc, float fW, float fH,
+                                                  float fD, float rx, float ry, float rz, float s0, float s1,
+                                                  float s2, float amin, float amax, float step) {
+  typename CellKey<IT>::type cell;
+  float xd, yd, zd;
+  float pix = 0.0f, vlast = 0.0f;
+  const int cap = GUARDED ? march_cap(amin, amax, step) : 0;
+  if (amin < amax) {  // entrance sample, weight 1/2
+    locate16<TS, HW, GUARDED, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH, fD,
+                                  cell, xd, yd, zd);
+    vlast = blend16<TS>(load16(v, cell), xd, yd, zd);
+    pix = fmaf(0.5f, vlast, pix);
+    amin += step;
+  }
+  int it = 0;
+  bool more = amin < amax && (!GUARDED || it < cap);
+  u32x4_a8 qn = {0u, 0u, 0u, 0u};
+  if (more) {
+    locate16<TS, HW, GUARDED, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH, fD,
+                                  cell, xd, yd, zd);
+    qn = load16(v, cell);
+  }
+  while (more) {
+    const u32x4_a8 qc = qn;
+    const float xc = xd, yc = yd, zc = zd;
+    amin += step;
+    ++it;
+    more = amin < amax && (!GUARDED || it < cap);
+    if (more) {  // the next sample's cell, requested before this sample is blended
+      locate16<TS, HW, GUARDED, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH,
+                                    fD, cell, xd, yd, zd);
+      qn = load16(v, cell);
+    }
+    vlast = blend16<TS>(qc, xc, yc, zc);
+    pix += vlast;
+  }
+  pix *= step;
+  if (pix > 0.0f) {  // tail only for a positive sum (cone_projector_3D_CudaKernel.cu.cc:164-179)
+    const float flast = vlast;
+    pix = fmaf(-(0.5f * step), flast, pix);
+    amin -= step;
+    const float last = amax - amin;
+    pix = fmaf(0.5f * last, flast, pix);
+    locate16<TS, HW, GUARDED, IT>(fmaf(amax, rx, s0), fmaf(amax, ry, s1), fmaf(amax, rz, s2), Dc, HDc, fW, fH, fD,
+                                  cell, xd, yd, zd);
+    pix = fmaf(0.5f * last, blend16<TS>(load16(v, cell), xd, yd, zd), pix);
+  }
+  return pix;
+}
+
 // The march of one ray from its clipped parameter interval: entrance sample with weight
 // 1/2, loop samples, and the tail rule (cone_projector_3D_CudaKernel.cu.cc:134-179).
 // GUARDED = false drops the coordinate clamp and the iteration cap; the caller uses it
@@ -501,8 +617,13 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
                      fmaxf(fabsf(s0), fmaxf(fabsf(s1), fabsf(s2))) < 0x1p18f && amag < 0x1p19f &&
                      amag * 0x1p-20f <= step;
   const float pix =
-      __all(plain) ? march_ray<TS, HW, false, IT, DIRECT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step)
-                   : march_ray<TS, HW, true, IT, DIRECT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step);
+      (sizeof(TS) == 2 && !DIRECT)
+          ? (__all(plain)
+                 ? march_ray_stream<TS, HW, false, IT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step)
+                 : march_ray_stream<TS, HW, true, IT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step))
+          : (__all(plain)
+                 ? march_ray<TS, HW, false, IT, DIRECT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step)
+                 : march_ray<TS, HW, true, IT, DIRECT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step));
   const float a = rx * spx, bq = ry * spy, c = rz * spz;
   *out = pix * sqrtf(fmaf(c, c, fmaf(bq, bq, a * a)));
 }
