@@ -410,24 +410,38 @@ __device__ __forceinline__ float march_ray_stream(const TS *__restrict__ v, IT D
   }
   int it = 0;
   bool more = amin < amax && (!GUARDED || it < cap);
-  u32x4_a8 qn = {0u, 0u, 0u, 0u};
+  // Two register sets, A and B, take turns: the loop body handles two samples, so that no
+  // cell or offsets move between registers from one sample to the next.
+  u32x4_a8 qa = {0u, 0u, 0u, 0u}, qb = {0u, 0u, 0u, 0u};
+  float xa = 0.0f, ya = 0.0f, za = 0.0f, xb = 0.0f, yb = 0.0f, zb = 0.0f;
   if (more) {
     locate16<TS, HW, GUARDED, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH, fD,
-                                  cell, xd, yd, zd);
-    qn = load16(v, cell);
+                                  cell, xa, ya, za);
+    qa = load16(v, cell);
   }
   while (more) {
-    const u32x4_a8 qc = qn;
-    const float xc = xd, yc = yd, zc = zd;
+    // the sample in A; the next sample's cell requested into B before A is blended
     amin += step;
     ++it;
     more = amin < amax && (!GUARDED || it < cap);
-    if (more) {  // the next sample's cell, requested before this sample is blended
+    if (more) {
       locate16<TS, HW, GUARDED, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH,
-                                    fD, cell, xd, yd, zd);
-      qn = load16(v, cell);
+                                    fD, cell, xb, yb, zb);
+      qb = load16(v, cell);
     }
-    vlast = blend16<TS>(qc, xc, yc, zc);
+    vlast = blend16<TS>(qa, xa, ya, za);
+    pix += vlast;
+    if (!more) break;
+    // the sample in B; the next one into A
+    amin += step;
+    ++it;
+    more = amin < amax && (!GUARDED || it < cap);
+    if (more) {
+      locate16<TS, HW, GUARDED, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH,
+                                    fD, cell, xa, ya, za);
+      qa = load16(v, cell);
+    }
+    vlast = blend16<TS>(qb, xb, yb, zb);
     pix += vlast;
   }
   pix *= step;
