@@ -191,7 +191,8 @@ __device__ __forceinline__ void unpack2(uint32_t w, _Float16, float &a, float &b
 // operand exactly inside the FMA, one rounding as before), so a load costs no
 // conversions at all.
 // IT is the index type of the cell array: int for arrays within the 32-bit limits of
-// fwd_small_index, long long beyond them.
+// fwd_small_index, long long beyond them.  The 16-bit streamed march also takes
+// IT = unsigned: 32-bit byte offsets from a biased base (locate16, fwd_byte_offsets).
 template <typename IT> struct CellKey { typedef unsigned type; };
 template <> struct CellKey<long long> { typedef unsigned long long type; };
 
@@ -352,7 +353,17 @@ __device__ __forceinline__ void locate16(float px, float py, float pz, IT Dc, IT
   xd = px - fx;
   yd = py - fy;
   zd = pz - fz;
-  if constexpr (sizeof(IT) == 4) {
+  if constexpr (std::is_same<IT, unsigned>::value) {
+    // Byte offset of the cell, two VALU instructions after the floors.  yz = fy * Dc + fz
+    // is an exact integer-valued float in [-(Dc + 1), 2^20); m = 8 yz + 2^23 + 8 (Dc + 1)
+    // is an exact integer in [2^23, 2^24) (the host keeps 8 HDc below 2^23), so its bit
+    // pattern is 0x4B000000 + m - 2^23, and the 24-bit multiply-add puts the x part on
+    // top.  The result, byte offset + 0x4B000000 - 8 HDc, lies in [0, 2^32) (the host
+    // keeps the cell array below 0xB5000000 bytes); cell_base subtracts the bias.
+    const float yz = fmaf(fy, (float)Dc, fz);
+    const float m = fmaf(yz, 8.0f, 0x1p23f + (float)(8u * (Dc + 1u)));
+    cell = (unsigned)__mul24((int)fx, (int)(8u * HDc)) + __float_as_uint(m);
+  } else if constexpr (sizeof(IT) == 4) {
     const int yz = (int)fmaf(fy, (float)Dc, fz);
     cell = (unsigned)(__mul24((int)fx, HDc) + yz + (HDc + Dc + 1));
   } else {
@@ -361,9 +372,23 @@ __device__ __forceinline__ void locate16(float px, float py, float pz, IT Dc, IT
   }
 }
 
-template <typename TS, typename K>
-__device__ __forceinline__ u32x4_a8 load16(const TS *__restrict__ cells, K cell) {
-  return *reinterpret_cast<const u32x4_a8 *>(reinterpret_cast<const uint2 *>(cells) + cell);
+// The address locate16's cell keys count from: the cell array itself, or for IT = unsigned
+// (byte offsets) the array moved by 8 HDc - 0x4B000000 bytes (wrapping 64-bit arithmetic;
+// every load adds the offset back).
+template <typename TS, typename IT>
+__device__ __forceinline__ const char *cell_base(const TS *cells, IT HDc) {
+  if constexpr (std::is_same<IT, unsigned>::value)
+    return reinterpret_cast<const char *>(cells) + (8ll * HDc - 0x4B000000ll);
+  else
+    return reinterpret_cast<const char *>(cells);
+}
+
+template <typename IT, typename K>
+__device__ __forceinline__ u32x4_a8 load16(const char *__restrict__ base, K cell) {
+  if constexpr (std::is_same<IT, unsigned>::value)
+    return *reinterpret_cast<const u32x4_a8 *>(base + cell);
+  else
+    return *reinterpret_cast<const u32x4_a8 *>(reinterpret_cast<const uint2 *>(base) + cell);
 }
 
 // The trilinear blend of sample_cells on a loaded 16-bit cell (quads at z and z + 1).
@@ -397,6 +422,7 @@ template <typename TS, bool HW, bool GUARDED, typename IT>
 __device__ __forceinline__ float march_ray_stream(const TS *__restrict__ v, IT Dc, IT HDc, float fW, float fH,
                                                   float fD, float rx, float ry, float rz, float s0, float s1,
                                                   float s2, float amin, float amax, float step) {
+  const char *vb = cell_base<TS, IT>(v, HDc);
   typename CellKey<IT>::type cell;
   float xd, yd, zd;
   float pix = 0.0f, vlast = 0.0f;
@@ -404,7 +430,7 @@ __device__ __forceinline__ float march_ray_stream(const TS *__restrict__ v, IT D
   if (amin < amax) {  // entrance sample, weight 1/2
     locate16<TS, HW, GUARDED, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH, fD,
                                   cell, xd, yd, zd);
-    vlast = blend16<TS>(load16(v, cell), xd, yd, zd);
+    vlast = blend16<TS>(load16<IT>(vb, cell), xd, yd, zd);
     pix = fmaf(0.5f, vlast, pix);
     amin += step;
   }
@@ -417,7 +443,40 @@ __device__ __forceinline__ float march_ray_stream(const TS *__restrict__ v, IT D
   if (more) {
     locate16<TS, HW, GUARDED, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH, fD,
                                   cell, xa, ya, za);
-    qa = load16(v, cell);
+    qa = load16<IT>(vb, cell);
+  }
+  if constexpr (!GUARDED) {
+    // Samples every lane of the wave is certain to take, marched first with no per-lane
+    // test: a wave-uniform count, so the loop test is scalar and no lane is masked.  Each
+    // alpha += step adds at most step + 2^-24 (|alpha| + step) while alpha stays within
+    // [amin, amax] (one rounding of at most 2^-24 of the sum), so j increments stay below
+    // amax while j (step + 2^-24 (M + step)) < amax - amin, M = max(|amin|, |amax|).  The
+    // count below takes 2^-22 for 2^-24, which also covers the roundings of its own three
+    // operations, and two increments less; the loop after it then runs the remaining
+    // samples with the test, so the samples and their order are those of the tested loop.
+    int safe = 0;
+    if (more) {
+      const float mag = fmaxf(fabsf(amin), fabsf(amax));
+      const float adv = step + (mag + step) * 0x1p-22f;
+      safe = (int)((amax - amin) / adv) - 2;
+    }
+    // (the reduction assumes every lane of the wave active, which fwd_block arranges; a
+    // wave with inactive lanes would skip this phase)
+    const int wmin = __ockl_wfred_min_i32(safe);
+    const int pairs = __builtin_amdgcn_read_exec() == ~0ull ? __builtin_amdgcn_readfirstlane(wmin) >> 1 : 0;
+    for (int k = 0; k < pairs; ++k) {
+      amin += step;
+      locate16<TS, HW, false, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH, fD,
+                                  cell, xb, yb, zb);
+      qb = load16<IT>(vb, cell);
+      pix += blend16<TS>(qa, xa, ya, za);
+      amin += step;
+      locate16<TS, HW, false, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH, fD,
+                                  cell, xa, ya, za);
+      qa = load16<IT>(vb, cell);
+      vlast = blend16<TS>(qb, xb, yb, zb);
+      pix += vlast;
+    }
   }
   while (more) {
     // the sample in A; the next sample's cell requested into B before A is blended
@@ -427,7 +486,7 @@ __device__ __forceinline__ float march_ray_stream(const TS *__restrict__ v, IT D
     if (more) {
       locate16<TS, HW, GUARDED, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH,
                                     fD, cell, xb, yb, zb);
-      qb = load16(v, cell);
+      qb = load16<IT>(vb, cell);
     }
     vlast = blend16<TS>(qa, xa, ya, za);
     pix += vlast;
@@ -439,7 +498,7 @@ __device__ __forceinline__ float march_ray_stream(const TS *__restrict__ v, IT D
     if (more) {
       locate16<TS, HW, GUARDED, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH,
                                     fD, cell, xa, ya, za);
-      qa = load16(v, cell);
+      qa = load16<IT>(vb, cell);
     }
     vlast = blend16<TS>(qb, xb, yb, zb);
     pix += vlast;
@@ -453,7 +512,7 @@ __device__ __forceinline__ float march_ray_stream(const TS *__restrict__ v, IT D
     pix = fmaf(0.5f * last, flast, pix);
     locate16<TS, HW, GUARDED, IT>(fmaf(amax, rx, s0), fmaf(amax, ry, s1), fmaf(amax, rz, s2), Dc, HDc, fW, fH, fD,
                                   cell, xd, yd, zd);
-    pix = fmaf(0.5f * last, blend16<TS>(load16(v, cell), xd, yd, zd), pix);
+    pix = fmaf(0.5f * last, blend16<TS>(load16<IT>(vb, cell), xd, yd, zd), pix);
   }
   return pix;
 }
@@ -562,10 +621,16 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   const long long w_ = qq - (long long)band * per_row;
   const int tx = (int)(w_ / P.n);
   const int view = (int)(w_ - (long long)tx * P.n);
-  const int col = tx * TL::TC + (wave % TL::WX) * TL::WC + lane / TL::WR;
-  const int row = ty * TL::TR + (wave / TL::WX) * TL::WR + lane % TL::WR;
   const int W = P.W, H = P.H, D = P.D, Hd = P.Hd, Wd = P.Wd;
-  if (col >= Wd || row >= Hd) return;
+  // Lanes past the detector's last column or row: in the 16-bit streamed march they march a
+  // copy of the edge pixel's ray and store nothing, so that every lane of the wave takes
+  // part in its wave-wide sample count; elsewhere they stop here.
+  const int col0 = tx * TL::TC + (wave % TL::WX) * TL::WC + lane / TL::WR;
+  const int row0 = ty * TL::TR + (wave / TL::WX) * TL::WR + lane % TL::WR;
+  const bool store = col0 < Wd && row0 < Hd;
+  if constexpr (sizeof(TS) == 4 || DIRECT)
+    if (!store) return;
+  const int col = col0 < Wd ? col0 : Wd - 1, row = row0 < Hd ? row0 : Hd - 1;
 
   const float *m = P.inv_ar + (size_t)view * 9;
   const float s0 = P.src[view * 3 + 0], s1 = P.src[view * 3 + 1], s2 = P.src[view * 3 + 2];
@@ -587,7 +652,7 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   rz *= nrm;
 
   if (!(step > 0.0f)) {
-    *out = __builtin_nanf("");
+    if (store) *out = __builtin_nanf("");
     return;
   }
 
@@ -630,16 +695,17 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
   const bool plain = rx != 0.0f && ry != 0.0f && rz != 0.0f &&
                      fmaxf(fabsf(s0), fmaxf(fabsf(s1), fabsf(s2))) < 0x1p18f && amag < 0x1p19f &&
                      amag * 0x1p-20f <= step;
-  const float pix =
-      (sizeof(TS) == 2 && !DIRECT)
-          ? (__all(plain)
-                 ? march_ray_stream<TS, HW, false, IT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step)
-                 : march_ray_stream<TS, HW, true, IT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step))
-          : (__all(plain)
-                 ? march_ray<TS, HW, false, IT, DIRECT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step)
-                 : march_ray<TS, HW, true, IT, DIRECT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step));
+  float pix;
+  if constexpr (sizeof(TS) == 2 && !DIRECT)
+    pix = __all(plain)
+              ? march_ray_stream<TS, HW, false, IT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step)
+              : march_ray_stream<TS, HW, true, IT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step);
+  else
+    pix = __all(plain)
+              ? march_ray<TS, HW, false, IT, DIRECT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step)
+              : march_ray<TS, HW, true, IT, DIRECT>(v, Wp, HWp, fW, fH, fD, rx, ry, rz, s0, s1, s2, amin, amax, step);
   const float a = rx * spx, bq = ry * spy, c = rz * spz;
-  *out = pix * sqrtf(fmaf(c, c, fmaf(bq, bq, a * a)));
+  if (store) *out = pix * sqrtf(fmaf(c, c, fmaf(bq, bq, a * a)));
 }
 
 // One launch covers the views P.n of a range of each item's n_all views (the host splits
@@ -993,6 +1059,13 @@ static bool fwd_small_index(int D, int H, int W) {
   return cells < (long long)INT32_MAX && W + 2 < (1 << 23) &&
          (long long)(H + 2) * (D + CellLayout<float>::kZExtra) < (1LL << 23);
 }
+// Whether the 16-bit forward may address its 8-byte cells by 32-bit byte offsets (locate16,
+// IT = unsigned): 8 (H + 2) Dc below 2^23 for the exact float and the 24-bit multiply, and
+// the cell array within 0xB5000000 bytes so that offset + 0x4B000000 stays below 2^32.
+static bool fwd_byte_offsets(int D, int H, int W) {
+  const long long plane = (long long)(H + 2) * (D + CellLayout<float>::kZExtra);
+  return W + 2 < (1 << 23) && 8 * plane < (1LL << 23) && 8 * (W + 2) * plane <= 0xB5000000LL;
+}
 static size_t det_cells(int n, int Hd, int Wd) { return (size_t)n * (Hd + 2) * (Wd + 2); }
 // Whether the backprojection may form detector cell offsets with one float FMA (exact below 2^24).
 static bool det_fidx(int Hd, int Wd) { return (long long)(Hd + 2) * (Wd + 2) <= (1LL << 24); }
@@ -1044,6 +1117,12 @@ static void launch_fwd(bool hw, const void *cells, float *sino, const float *inv
     FwdProblem P{cells, sino + (size_t)v0 * Hd * Wd, inv_ar + (size_t)v0 * 9, src + (size_t)v0 * 3, step,
                  spacing, W, H, D, k, Hd, Wd, (Wd + 15) / 16, (Hd + 15) / 16, 0};
     dim3 grid((unsigned)fwd_blocks(k, Hd, Wd), 1, B);
+    if constexpr (sizeof(TS) == 2) {
+      if (fwd_byte_offsets(D, H, W)) {
+        launch_fwd_it<TS, unsigned>(hw, P, grid, cs, n, stream);
+        continue;
+      }
+    }
     if (small)
       launch_fwd_it<TS, int>(hw, P, grid, cs, n, stream);
     else

@@ -105,6 +105,33 @@ def test_forward_64bit_cell_index_wide_plane(storage, hw):
     _check_views(vol, P, sp, og, det, 0.5, hw, views=(0, 1))
 
 
+@pytest.mark.parametrize("storage", [torch.bfloat16, torch.float16])
+def test_forward_16bit_element_index_plane_2_20(storage):
+    """(H + 2)(D + 3) = 2^20 cells per x plane: too wide for the 16-bit forward's 32-bit byte
+    offsets (8 per cell below 2^23), so it takes the 32-bit element index."""
+    shape = (1021, 1022, 2)
+    assert (shape[1] + 2) * (shape[0] + 3) == 2 ** 20
+    det = (64, 64)
+    sp, og, P = _orbit(shape, 4, 3000.0, 5000.0, det, (20.0, 0.5))
+    print(f"16-bit element index {shape} {storage}", flush=True)
+    vol = _rand_volume(shape, 5, storage)
+    _check_views(vol, P, sp, og, det, 0.5, False, views=(0, 1))
+
+
+@pytest.mark.parametrize("storage", [torch.bfloat16, torch.float16])
+def test_forward_16bit_byte_offsets_at_their_limit(storage):
+    """1000 x 1020 x 368: 3.03e9 bytes of 16-bit cells, just within the byte offsets' limit
+    (offset + 0x4B000000 below 2^32); view 0's rays run along x through the whole array."""
+    shape = (1000, 1020, 368)
+    plane = (shape[1] + 2) * (shape[0] + 3)
+    assert 8 * plane < 2 ** 23 and 0xB5000000 - 8 * plane < 8 * (shape[2] + 2) * plane <= 0xB5000000
+    det = (32, 48)
+    sp, og, P = _orbit(shape, 4, 3000.0, 5000.0, det, (30.0, 12.0))
+    print(f"16-bit byte offsets {shape} {storage}", flush=True)
+    vol = _rand_volume(shape, 6, storage)
+    _check_views(vol, P, sp, og, det, 0.5, False, views=(0, 1))
+
+
 def test_forward_more_than_2_31_cells():
     """32 x 8192 x 8192 (2.1e9 voxels, below the reference's 2^32): 2.35e9 cells, BF16 storage."""
     shape = (32, 8192, 8192)
