@@ -463,19 +463,41 @@ __device__ __forceinline__ float march_ray_stream(const TS *__restrict__ v, IT D
     // (the reduction assumes every lane of the wave active, which fwd_block arranges; a
     // wave with inactive lanes would skip this phase)
     const int wmin = __ockl_wfred_min_i32(safe);
-    const int pairs = __builtin_amdgcn_read_exec() == ~0ull ? __builtin_amdgcn_readfirstlane(wmin) >> 1 : 0;
-    for (int k = 0; k < pairs; ++k) {
+    const int nsafe = __builtin_amdgcn_read_exec() == ~0ull ? __builtin_amdgcn_readfirstlane(wmin) : 0;
+    if (nsafe >= 4) {
+      // Three register sets, A, B and C, take turns, so that two loads are in flight while
+      // a sample is blended: B is requested first, then each step requests the sample two
+      // ahead and blends the oldest.  1 + 3 t increments, t = (nsafe - 1) / 3.
+      u32x4_a8 qc;
+      float xc, yc, zc;
       amin += step;
       locate16<TS, HW, false, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH, fD,
                                   cell, xb, yb, zb);
       qb = load16<IT>(vb, cell);
+      const int triples = (nsafe - 1) / 3;
+      for (int k = 0; k < triples; ++k) {
+        amin += step;
+        locate16<TS, HW, false, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH,
+                                    fD, cell, xc, yc, zc);
+        qc = load16<IT>(vb, cell);
+        pix += blend16<TS>(qa, xa, ya, za);
+        amin += step;
+        locate16<TS, HW, false, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH,
+                                    fD, cell, xa, ya, za);
+        qa = load16<IT>(vb, cell);
+        pix += blend16<TS>(qb, xb, yb, zb);
+        amin += step;
+        locate16<TS, HW, false, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH,
+                                    fD, cell, xb, yb, zb);
+        qb = load16<IT>(vb, cell);
+        pix += blend16<TS>(qc, xc, yc, zc);
+      }
+      // A and B hold the next two samples and alpha is B's: blend A, and B becomes A
       pix += blend16<TS>(qa, xa, ya, za);
-      amin += step;
-      locate16<TS, HW, false, IT>(fmaf(amin, rx, s0), fmaf(amin, ry, s1), fmaf(amin, rz, s2), Dc, HDc, fW, fH, fD,
-                                  cell, xa, ya, za);
-      qa = load16<IT>(vb, cell);
-      vlast = blend16<TS>(qb, xb, yb, zb);
-      pix += vlast;
+      qa = qb;
+      xa = xb;
+      ya = yb;
+      za = zb;
     }
   }
   while (more) {
