@@ -1594,7 +1594,7 @@ static int fwd_batched(const pyronn_cone_problem *pr, int count, bool hw, int bp
   ws += kCounterBytes;
   FwdBatch fb{};
   long long begin = 0;
-  bool small = true;
+  bool small = true, bytes = sizeof(TS) == 2;
   const int tc = fwd_tile_cols(tile), tr = fwd_tile_rows(tile);
   for (int k = 0; k < count; ++k) {
     const pyronn_cone_problem &p = pr[k];
@@ -1606,6 +1606,7 @@ static int fwd_batched(const pyronn_cone_problem *pr, int count, bool hw, int bp
         p.projection_matrices, p.volume_origin, p.volume_spacing, 1, p.n_views, inv_ar, src);
     launch_cells<TS>(p.input, cells, 1, p.D, p.H, p.W, s);
     small = small && fwd_small_index(p.D, p.H, p.W);
+    bytes = bytes && fwd_byte_offsets(p.D, p.H, p.W);
     fb.p[k] = FwdProblem{cells, p.output, inv_ar, src, p.step_size, p.volume_spacing,
                          p.W, p.H, p.D, p.n_views, p.det_h, p.det_w,
                          (p.det_w + tc - 1) / tc, (p.det_h + tr - 1) / tr, begin};
@@ -1618,6 +1619,13 @@ static int fwd_batched(const pyronn_cone_problem *pr, int count, bool hw, int bp
   if (grid > begin) grid = begin;
   reset_counters_kernel<<<1, 64, 0, s>>>(counters);
   const dim3 g((unsigned)grid);
+  if constexpr (sizeof(TS) == 2) {
+    if (bytes) {
+      if (hw) launch_fwd_persistent<TS, true, unsigned>(fb, tile, g, counters, s);
+      else launch_fwd_persistent<TS, false, unsigned>(fb, tile, g, counters, s);
+      return PYRONN_OK;
+    }
+  }
   if (small) {
     if (hw) launch_fwd_persistent<TS, true, int>(fb, tile, g, counters, s);
     else launch_fwd_persistent<TS, false, int>(fb, tile, g, counters, s);
