@@ -68,9 +68,10 @@ def test_cone_random_geometry_bit_exact(seed):
         assert np.array_equal(g, r), (seed, hw, shape, det, step, int((g != r).sum()))
     for storage in ("bf16", "f16"):      # 16-bit storage: the oracle on the values the kernel reads
         vq = gh.quantise(vol, storage)
-        g = gh.cone_forward(vol, P, sp, og, det, step, storage=storage)[0]
-        r, _, _ = orc.cone_forward(vq, inv, src, det, sp, step)
-        assert np.array_equal(g, r), (seed, storage, int((g != r).sum()))
+        for hw in (False, True):
+            g = gh.cone_forward(vol, P, sp, og, det, step, hw=hw, storage=storage)[0]
+            r, _, _ = orc.cone_forward(vq, inv, src, det, sp, step, hw=hw)
+            assert np.array_equal(g, r), (seed, storage, hw, int((g != r).sum()))
     rng = np.random.default_rng(1000 + seed)
     sino = rng.uniform(-1, 2, (P.shape[0],) + det).astype(np.float32)
     for hw in (False, True):
