@@ -113,6 +113,23 @@ def test_c3_cone_256_views_and_slices(hw):
         assert np.array_equal(got, rb), f"slices {z0}..{z0 + 1} hw={hw}: not bit-identical to the oracle"
 
 
+@pytest.mark.parametrize("storage", [torch.bfloat16, torch.float16])
+def test_c3_cone_256_16bit_forward(storage):
+    """The 256^3, 360-view, 400 x 400 forward with 16-bit volume storage (the streamed march on
+    32-bit byte offsets) against the oracle run on the same values widened to float32."""
+    import pyronn_layers as pl
+    pl.load()
+    shape, det = (256, 256, 256), (400, 400)
+    sp, og, P, vol, inv, src = _cone_setup(shape, det, (1.0, 1.0), 360, 750.0, 1200.0, 3, (8.0, 100.0))
+    v16 = vol.to(storage)
+    sino = _fwd(v16, P, sp, og, det, 0.5)
+    vh = v16.float().cpu().numpy()
+    for v0 in (0, 45, 263):
+        r, _, _ = orc.cone_forward(vh, inv, src, det, sp, 0.5, views=(v0, v0 + 1))
+        got = sino[v0:v0 + 1].cpu().numpy()
+        assert np.array_equal(got, r), f"view {v0} {storage}: not bit-identical to the oracle"
+
+
 def test_m1_cone_512_subsets_and_properties():
     import pyronn_layers as pl
     pl.load()
