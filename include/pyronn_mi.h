@@ -305,9 +305,11 @@ int pyronn_fdk_filter(const void *sinogram, int sinogram_dtype, int images, int 
  * with h(c - j), |c - j| < det_w, that T holds.  det_w is limited to 1..4096.
  *
  * pyronn_fdk_ramp_spectrum_size: floats of the filter table for det_w (0 outside
- * the limit).  pyronn_fdk_ramp_spectrum: builds the table (device, 8-byte aligned):
+ * the limit).  pyronn_fdk_ramp_spectrum: builds the table (device, 16-byte aligned):
  * the ramp spectrum scaled by tau / L, the FFT twiddles and the per-output-sample
- * column weights (NULL: all 1), the FFT counterpart of pyronn_fdk_ramp_matrix.
+ * column weights (NULL: all 1), the FFT counterpart of pyronn_fdk_ramp_matrix.  The
+ * table's layout is private to the library; a table is valid for the det_w it was
+ * built for.
  *
  * pyronn_fdk_filter_fft: filtered[i][r][:] = ((weights[r][:] .* sinogram[i][r][:])
  * conv h) .* column_weights, stored as filtered_dtype; no workspace.  Rows 2p and

@@ -206,17 +206,24 @@ def test_batched_argument_validation():
 
 def test_fft_filter_limits_and_validation():
     L = nat.lib()
-    # table: ramp spectrum (L) + twiddles (2 L) + column weights (W), L = pow2 >= max(16, 2 W - 1)
-    assert L.pyronn_fdk_ramp_spectrum_size(1) == 3 * 16 + 1
-    assert L.pyronn_fdk_ramp_spectrum_size(9) == 3 * 32 + 9
-    assert L.pyronn_fdk_ramp_spectrum_size(1024) == 3 * 2048 + 1024
-    assert L.pyronn_fdk_ramp_spectrum_size(4096) == 3 * 8192 + 4096
+    # table floats: ramp spectrum (L) + 4 per twiddle pair + column weights (W), L = pow2 >=
+    # max(16, 2 W - 1); one twiddle pair per butterfly index k < Ns of each pass with Ns > 1,
+    # forward and inverse (radix 2 or 4 first when log2 L is not a multiple of 3, radix 8 after):
+    #   L = 16   (2, 8):        forward Ns 2; inverse (8, 2) Ns 8                     -> 10 pairs
+    #   L = 32   (4, 8):        forward Ns 4; inverse Ns 8                            -> 12
+    #   L = 2048 (4, 8, 8, 8):  forward Ns 4, 32, 256; inverse Ns 8, 64, 512          -> 876
+    #   L = 8192 (2, 8, 8, 8, 8): forward 2, 16, 128, 1024; inverse 8, 64, 512, 4096  -> 5850
+    assert L.pyronn_fdk_ramp_spectrum_size(1) == 16 + 4 * 10 + 1
+    assert L.pyronn_fdk_ramp_spectrum_size(9) == 32 + 4 * 12 + 9
+    assert L.pyronn_fdk_ramp_spectrum_size(1024) == 2048 + 4 * 876 + 1024
+    assert L.pyronn_fdk_ramp_spectrum_size(4096) == 8192 + 4 * 5850 + 4096
     assert L.pyronn_fdk_ramp_spectrum_size(4097) == 0
     assert L.pyronn_fdk_ramp_spectrum_size(0) == 0
     assert L.pyronn_fdk_ramp_spectrum(4097, 1.0, None, _fake(), None) == 1 and "outside" in nat.last_error()
     assert L.pyronn_fdk_ramp_spectrum(64, 0.0, None, _fake(), None) == 1 and "spacing" in nat.last_error()
-    assert L.pyronn_fdk_ramp_spectrum(64, 1.0, None, ctypes.c_void_p(0x1004), None) == 1
-    assert "aligned" in nat.last_error()
+    for addr in (0x1004, 0x1008):   # the table is read with 16-byte loads
+        assert L.pyronn_fdk_ramp_spectrum(64, 1.0, None, ctypes.c_void_p(addr), None) == 1
+        assert "16-byte aligned" in nat.last_error()
     args = lambda imgs, h, w, dt: (_fake(), dt, imgs, h, w, None, _fake(2), _fake(3), 0, None)
     assert L.pyronn_fdk_filter_fft(*args(1, 4, 5000, 0)) == 1 and "outside" in nat.last_error()
     assert L.pyronn_fdk_filter_fft(*args(0, 4, 64, 0)) == 1 and "shape" in nat.last_error()
