@@ -103,6 +103,37 @@ def test_fft_filter_matches_convolution_and_gemm(det_w, det_h):
         assert torch.equal(one, out[i:i + 1])
 
 
+@pytest.mark.parametrize("dt", [torch.float32, torch.bfloat16])
+def test_fft_filter_vector_and_scalar_access_agree(dt):
+    """Rows whose width is a multiple of 4 in aligned buffers take one vector load / store per
+    four samples; the same rows at an address 1 element off take the scalar accesses.  The
+    arithmetic is the same, so the bits are."""
+    from pyronn_layers import native as nat
+    from pyronn_layers.fdk import _DT, _p, _stream
+    rng = np.random.default_rng(11)
+    images, det_h, det_w = 3, 5, 256
+    code = _DT[dt]
+    x = torch.tensor(rng.normal(size=(images, det_h, det_w)).astype(np.float32), device="cuda").to(dt)
+    wts = torch.tensor(rng.uniform(0.5, 1.0, (det_h, det_w)).astype(np.float32), device="cuda")
+    spec = torch.empty(nat.lib().pyronn_fdk_ramp_spectrum_size(det_w), device="cuda")
+    nat.call("pyronn_fdk_ramp_spectrum", det_w, 0.7, None, _p(spec), _stream())
+    n = x.numel()
+    aligned = torch.empty(n, dtype=dt, device="cuda")
+    nat.call("pyronn_fdk_filter_fft", _p(x), code, images, det_h, det_w, _p(wts), _p(spec), _p(aligned), code,
+             _stream())
+    xs = torch.empty(n + 1, dtype=dt, device="cuda")
+    xs[1:] = x.reshape(-1)
+    ws = torch.empty(det_h * det_w + 1, device="cuda")
+    ws[1:] = wts.reshape(-1)
+    outs = torch.zeros(n + 1, dtype=dt, device="cuda")
+    for pin, pw, po in ((xs[1:], wts, aligned.new_empty(n)), (x, ws[1:], aligned.new_empty(n)), (x, wts, outs[1:])):
+        assert pin.data_ptr() % 16 or pw.data_ptr() % 16 or po.data_ptr() % 8
+        nat.call("pyronn_fdk_filter_fft", _p(pin), code, images, det_h, det_w, _p(pw), _p(spec), _p(po), code,
+                 _stream())
+        torch.cuda.synchronize()
+        assert torch.equal(po, aligned)
+
+
 @pytest.mark.parametrize("src,dst", [(torch.bfloat16, torch.float32), (torch.float32, torch.bfloat16),
                                      (torch.float16, torch.float16), (torch.bfloat16, torch.bfloat16)])
 def test_fft_filter_storage_types(src, dst):
