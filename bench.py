@@ -738,7 +738,7 @@ def batched_bench(args):
         tb = tuner.backward(sin, geoms)
         _, t_bp = timed(lambda: batched.cone_backprojection3d_batched(sin, geoms, z_per_lane=tb["z_per_lane"],
                                                                       blocks_per_cu=tb["blocks_per_cu"]))
-        # the same problems launched one after another through the single-problem ops
+        # the same problems, one batched launch per problem, one after another
         def seq_fwd():
             for v, g in zip(vin, geoms):
                 batched.cone_projection3d_batched([v], [g], blocks_per_cu=tf["blocks_per_cu"],

@@ -245,7 +245,8 @@ typedef struct pyronn_cone_problem {
 size_t pyronn_cone_projection3d_batched_workspace_size(const pyronn_cone_problem *problems, int count,
                                                        int dtype);
 /* Tuning parameters (results are the same bits for every value):
- * blocks_per_cu: persistent workgroups per CU (<= 0: 8);
+ * blocks_per_cu > 0: a persistent grid of that many workgroups per CU claiming items from
+ *   per-XCD counters; <= 0 (default): one workgroup per item in item order (no claims);
  * forward_tile: the detector tile of a 256-lane workgroup and the waves within it,
  *   0: 16 x 16 pixels, waves of 4 columns x 16 rows (the single-problem shape),
  *   1: 8 x 32, waves 2 x 32;  2: 32 x 8, waves 8 x 8;  3: 16 x 16, waves 8 x 8. */

@@ -801,6 +801,18 @@ cone_fwd_persistent_kernel(FwdBatch B, unsigned long long *counters) {
   }
 }
 
+// The same work list as one workgroup per item in item order, from item item0 (a multiple
+// of 8) on: the hardware deals workgroups round-robin over the XCDs in block order, so item
+// mod 8 is the XCD, as in the single-problem grid, and no claim (atomic and two barriers)
+// precedes a block.
+template <typename TS, bool HW, typename IT, typename TL>
+__global__ void __launch_bounds__(256) cone_fwd_items_kernel(FwdBatch B, long long item0) {
+  const long long w = item0 + blockIdx.x;
+  int k = 0;
+  while (k + 1 < B.count && w >= B.p[k + 1].begin) ++k;
+  fwd_block<TS, HW, IT, false, TL>(B.p[k], w - B.p[k].begin);
+}
+
 // =====================================================================================
 // Detector cells for the backprojection: per view, cell (u, v), u in [-1, W_d],
 // v in [-1, H_d], holds the bilinear neighbourhood
@@ -1576,9 +1588,24 @@ static size_t fwd_problem_bytes(const pyronn_cone_problem &p, int dtype) {
 
 constexpr size_t kCounterBytes = 256;  // per-XCD work counters at the start of a batched workspace
 
+// g.x > 0: a persistent grid of g.x workgroups claiming items; g.x == 0: one workgroup per
+// item, in launches of at most kMaxGridX items (a multiple of 8).
 template <typename TS, bool HW, typename IT>
 static void launch_fwd_persistent(const FwdBatch &fb, int tile, dim3 g, unsigned long long *counters,
                                   hipStream_t s) {
+  if (g.x == 0) {
+    const long long per = kMaxGridX / kXcds * kXcds;
+    for (long long i0 = 0; i0 < fb.total; i0 += per) {
+      const dim3 gi((unsigned)(fb.total - i0 < per ? fb.total - i0 : per));
+      switch (tile) {
+        case 1: cone_fwd_items_kernel<TS, HW, IT, FwdTile<8, 32, 2>><<<gi, dim3(256), 0, s>>>(fb, i0); break;
+        case 2: cone_fwd_items_kernel<TS, HW, IT, FwdTile<32, 8, 8>><<<gi, dim3(256), 0, s>>>(fb, i0); break;
+        case 3: cone_fwd_items_kernel<TS, HW, IT, FwdTile<16, 16, 8>><<<gi, dim3(256), 0, s>>>(fb, i0); break;
+        default: cone_fwd_items_kernel<TS, HW, IT, FwdTile16><<<gi, dim3(256), 0, s>>>(fb, i0); break;
+      }
+    }
+    return;
+  }
   switch (tile) {
     case 1: cone_fwd_persistent_kernel<TS, HW, IT, FwdTile<8, 32, 2>><<<g, dim3(256), 0, s>>>(fb, counters); break;
     case 2: cone_fwd_persistent_kernel<TS, HW, IT, FwdTile<32, 8, 8>><<<g, dim3(256), 0, s>>>(fb, counters); break;
@@ -1614,10 +1641,14 @@ static int fwd_batched(const pyronn_cone_problem *pr, int count, bool hw, int bp
   }
   fb.count = count;
   fb.total = begin;
-  long long grid = (long long)device_cus() * (bpc > 0 ? bpc : 8);
-  grid = (grid + kXcds - 1) / kXcds * kXcds;
-  if (grid > begin) grid = begin;
-  reset_counters_kernel<<<1, 64, 0, s>>>(counters);
+  // bpc <= 0: one workgroup per item (grid 0 below); else a persistent grid
+  long long grid = 0;
+  if (bpc > 0) {
+    grid = (long long)device_cus() * bpc;
+    grid = (grid + kXcds - 1) / kXcds * kXcds;
+    if (grid > begin) grid = begin;
+    reset_counters_kernel<<<1, 64, 0, s>>>(counters);
+  }
   const dim3 g((unsigned)grid);
   if constexpr (sizeof(TS) == 2) {
     if (bytes) {

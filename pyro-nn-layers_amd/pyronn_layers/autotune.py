@@ -3,8 +3,8 @@
 Tunable parameters (all preserve results bit for bit):
   forward          forward_tile in {0, 1, 2, 3}: the detector tile of a workgroup and its
                    waves (16x16 of 4x16 waves, 8x32 of 2x32, 32x8 of 8x8, 16x16 of 8x8),
-                   timed at 8 workgroups per CU; then blocks_per_cu in {4, 16} for the
-                   best tile (persistent grid = CUs x value)
+                   timed with one workgroup per item; then blocks_per_cu in {8, 16} for the
+                   best tile (a persistent grid of CUs x value workgroups)
   backprojection   z_per_lane in {4, 8} x blocks_per_cu in {0, 4, 8, 16}
                    (0: one workgroup per item; otherwise a persistent grid)
 Each candidate is timed with HIP events (best of `reps` after one warm-up) and
@@ -81,9 +81,9 @@ class Autotuner:
                     volumes, geometries, hardware_interp, blocks_per_cu=bpc, forward_tile=tile), self.reps)
 
             for tile in TILE_SHAPES:
-                trial(8, tile)
+                trial(0, tile)
             tile = min(trials, key=trials.get)[1]
-            for bpc in (4, 16):
+            for bpc in (8, 16):
                 trial(bpc, tile)
             best = min(trials, key=trials.get)
             self.cache[key] = {"blocks_per_cu": best[0], "forward_tile": best[1], "tile_shape": TILE_SHAPES[best[1]],

@@ -1,7 +1,7 @@
 """Uneven batches: cone projections / backprojections of problems with different
 volume sizes, detectors and geometries in one launch each
-(pyronn_cone_projection3d_batched: a persistent grid; pyronn_cone_backprojection3d_batched:
-one workgroup per item, or a persistent grid with blocks_per_cu > 0).
+(pyronn_cone_projection3d_batched, pyronn_cone_backprojection3d_batched: one workgroup per
+item, or a persistent grid with blocks_per_cu > 0).
 
 Every output equals the single-problem operator's output bit for bit; the
 batched launch only changes which workgroup computes which tile.

@@ -56,7 +56,8 @@ def test_batched_equals_single(storage, hw):
     geoms, vols = _problems()
     vt = [torch.tensor(v, device="cuda").to(storage) for v in vols]
     singles = [_single_fwd(v, g, hw) for v, g in zip(vt, geoms)]
-    for bpc, tile in ((0, 0), (2, 0), (4, 1), (8, 2), (16, 3)):   # every forward tile shape
+    # one workgroup per item (blocks_per_cu 0) and persistent grids, every forward tile shape
+    for bpc, tile in ((0, 0), (0, 1), (0, 2), (0, 3), (-1, 0), (2, 0), (4, 1), (8, 2), (16, 3)):
         sinos = batched.cone_projection3d_batched(vt, geoms, hw, blocks_per_cu=bpc, forward_tile=tile)
         for s, s1 in zip(sinos, singles):
             assert torch.equal(s, s1), f"blocks_per_cu {bpc} tile {tile}"
@@ -75,7 +76,7 @@ def test_autotuner_caches_choice(tmp_path):
     vt = [torch.tensor(v, device="cuda") for v in vols]
     at = Autotuner(str(tmp_path / "tune.json"), reps=1)
     f = at.forward(vt, geoms)
-    assert f["blocks_per_cu"] in (4, 8, 16)
+    assert f["blocks_per_cu"] in (0, 8, 16)
     assert f["forward_tile"] in (0, 1, 2, 3) and "x" in f["tile_shape"]
     assert len(f["trials_ms"]) == 6                      # four tiles, then two more grid sizes
     from pyronn_layers import batched
