@@ -25,7 +25,7 @@ def _f32(a, dev):
 
 
 class _Geo:
-    """Device copies of one ConeGeometry's arrays (kept alive during a call)."""
+    """Device copies of one ConeGeometry's arrays (P, vo, vs, step, mult; g the geometry)."""
 
     def __init__(self, g: geo.ConeGeometry, dev):
         self.g = g
@@ -72,19 +72,29 @@ def _check(n):
         raise ValueError(f"1..{nat.MAX_BATCHED_PROBLEMS} problems per batched launch, got {n}")
 
 
+def device_geometry(geometries: Sequence[geo.ConeGeometry], device) -> list:
+    """The device arrays of each geometry, to keep resident across batched calls
+    (device_geometry= of cone_projection3d_batched / cone_backprojection3d_batched), so that a
+    call uploads no projection matrices."""
+    dev = torch.device(device)
+    return [_Geo(g, dev) for g in geometries]
+
+
 def cone_projection3d_batched(volumes: Sequence[torch.Tensor], geometries: Sequence[geo.ConeGeometry],
                               hardware_interp: bool = False, blocks_per_cu: int = 0,
-                              forward_tile: int = 0) -> List[torch.Tensor]:
+                              forward_tile: int = 0, device_geometry=None) -> List[torch.Tensor]:
     """volumes[k] [D_k, H_k, W_k] (one storage dtype for all) -> sinograms [n_k, H_d,k, W_d,k] float32.
     blocks_per_cu and forward_tile (0..3: 16x16, 8x32, 32x8 and 16x16 with 8x8 waves) are launch
-    shapes that never change a result."""
+    shapes that never change a result.  device_geometry: optional per-problem device arrays
+    already resident (as for cone_backprojection3d_batched)."""
     _validate(volumes, geometries, True)
     dev = volumes[0].device
     dt = volumes[0].dtype
     if any(v.dtype != dt for v in volumes):
         raise TypeError("all volumes of a batched launch share one storage dtype")
     vols = [v.contiguous() for v in volumes]
-    geos = [_Geo(g, dev) for g in geometries]
+    geos = (_device_geometry(device_geometry, geometries, dev) if device_geometry is not None
+            else [_Geo(g, dev) for g in geometries])
     outs = [torch.empty((g.n_views,) + tuple(g.detector_shape), dtype=torch.float32, device=dev) for g in geometries]
     arr = _problems(vols, outs, geos)
     L = nat.lib()

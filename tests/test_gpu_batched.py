@@ -66,6 +66,12 @@ def test_batched_equals_single(storage, hw):
         recs = batched.cone_backprojection3d_batched(st, geoms, hw, z_per_lane=zpl, blocks_per_cu=bpc)
         for r, s, g in zip(recs, st, geoms):
             assert torch.equal(r, _single_bp(s, g, hw))
+    # geometry arrays kept resident on the device across calls
+    dg = batched.device_geometry(geoms, "cuda")
+    for s, s1 in zip(batched.cone_projection3d_batched(vt, geoms, hw, device_geometry=dg), singles):
+        assert torch.equal(s, s1)
+    for r, r1 in zip(batched.cone_backprojection3d_batched(st, geoms, hw, device_geometry=dg), recs):
+        assert torch.equal(r, r1)
 
 
 def test_autotuner_caches_choice(tmp_path):
