@@ -8,10 +8,20 @@
 // b - i (exact, not 8-bit), neighbours outside the array read 0.
 //
 // The 2D problems are small (a 512^2 image is 1 MiB), so the samplers test
-// bounds per neighbour instead of reading a zero-apron copy.
+// bounds per neighbour instead of reading a zero-apron copy.  With f32 storage the two
+// x neighbours of a row are adjacent: when both lie inside the row, one 8-byte load reads
+// them (dword aligned; the target's vector loads do not need natural alignment), which
+// halves the loads of a sample without changing any value.
 #include "common.h"
 
 namespace pyronn {
+
+typedef float f32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+
+// img[i], img[i + 1] as one load
+__device__ __forceinline__ f32x2_a4 load_pair(const float *__restrict__ img, size_t i) {
+  return *reinterpret_cast<const f32x2_a4 *>(img + i);
+}
 
 constexpr float kPi = 3.14159265359f;  // par_backprojector_2D_CudaKernel.cu.cc:32
 
@@ -54,10 +64,20 @@ __device__ __forceinline__ float tex2d_cached(const TS *__restrict__ img, int W,
     const int x1 = x0 + 1, y1 = y0 + 1;
     const bool vx0 = x0 >= 0 && x0 < W, vx1 = x1 >= 0 && x1 < W;
     const bool vy0 = y0 >= 0 && y0 < H, vy1 = y1 >= 0 && y1 < H;
-    r.p00 = (vx0 && vy0) ? to_f32(img[y0 * W + x0]) : 0.0f;
-    r.p01 = (vx1 && vy0) ? to_f32(img[y0 * W + x1]) : 0.0f;
-    r.p10 = (vx0 && vy1) ? to_f32(img[y1 * W + x0]) : 0.0f;
-    r.p11 = (vx1 && vy1) ? to_f32(img[y1 * W + x1]) : 0.0f;
+    if (sizeof(TS) == 4 && vx0 && vx1) {
+      const float *f = reinterpret_cast<const float *>(img);
+      const f32x2_a4 a = vy0 ? load_pair(f, (size_t)y0 * W + x0) : f32x2_a4{0.0f, 0.0f};
+      const f32x2_a4 b = vy1 ? load_pair(f, (size_t)y1 * W + x0) : f32x2_a4{0.0f, 0.0f};
+      r.p00 = a.x;
+      r.p01 = a.y;
+      r.p10 = b.x;
+      r.p11 = b.y;
+    } else {
+      r.p00 = (vx0 && vy0) ? to_f32(img[y0 * W + x0]) : 0.0f;
+      r.p01 = (vx1 && vy0) ? to_f32(img[y0 * W + x1]) : 0.0f;
+      r.p10 = (vx0 && vy1) ? to_f32(img[y1 * W + x0]) : 0.0f;
+      r.p11 = (vx1 && vy1) ? to_f32(img[y1 * W + x1]) : 0.0f;
+    }
   }
   const float xd = bx - fx, yd = by - fy;
   const float p0 = lerp_ref(r.p00, r.p01, xd);
@@ -71,8 +91,15 @@ __device__ __forceinline__ float tex1d(const TS *__restrict__ row, int W, float 
   if (!(bx > -2.0f && bx < (float)(W + 1))) return 0.0f;
   const float fx = floorf(bx);
   const int x0 = (int)fx, x1 = x0 + 1;
-  const float p0 = (x0 >= 0 && x0 < W) ? to_f32(row[x0]) : 0.0f;
-  const float p1 = (x1 >= 0 && x1 < W) ? to_f32(row[x1]) : 0.0f;
+  float p0, p1;
+  if (sizeof(TS) == 4 && x0 >= 0 && x1 < W) {
+    const f32x2_a4 a = load_pair(reinterpret_cast<const float *>(row), (size_t)x0);
+    p0 = a.x;
+    p1 = a.y;
+  } else {
+    p0 = (x0 >= 0 && x0 < W) ? to_f32(row[x0]) : 0.0f;
+    p1 = (x1 >= 0 && x1 < W) ? to_f32(row[x1]) : 0.0f;
+  }
   return lerp_ref(p0, p1, bx - fx);
 }
 
