@@ -65,6 +65,8 @@ def main(argv=None):
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--serve-chunks", type=int, default=8,
                     help="view ranges of a 512^3 request uploaded while earlier ranges reconstruct (1 = off)")
+    ap.add_argument("--serve-chunks-256", type=int, default=1,
+                    help="the same for the cone 256^3 requests (1 = off: one upload, then one HIP graph)")
     ap.add_argument("--serve-curve", type=lambda t: [float(x) for x in t.split(",") if x], default=[16.0, 32.0, 48.0, 64.0],
                     help="offered loads (req/s) of the serving latency curve, comma separated")
     ap.add_argument("--no-serve-projections", dest="serve_projections", action="store_false",
@@ -378,7 +380,8 @@ def serve_bench(args):
                        "(sigma = 0.01 max) from pinned host memory; latency = arrival -> volume in host memory"
                        % (args.serve_rate, args.serve_requests),
            "slots": args.serve_slots, "hip_graphs": True,
-           "cone512_view_ranges": args.serve_chunks if args.serve_chunks > 1 else None}
+           "cone512_view_ranges": args.serve_chunks if args.serve_chunks > 1 else None,
+           "cone256_view_ranges": args.serve_chunks_256 if args.serve_chunks_256 > 1 else None}
     for name, storage in (("f32", torch.float32), ("bf16", torch.bfloat16)):
         payloads = {}
         for k, s_ in clean.items():
@@ -404,7 +407,7 @@ def serve_bench(args):
         log(f"serving {name}: 512^3 ramp filter FFT {filt['fft_ms']:.2f} ms, GEMM {filt['gemm_ms']:.2f} ms")
         p512.ramp = None
         del x
-        chunked = {"cone512": args.serve_chunks} if args.serve_chunks > 1 else {}
+        chunked = _chunked(args)
         ex = GpuExecutor(plans, slots=args.serve_slots, use_graphs=True, chunked=chunked,
                          group_kinds=("cone256",))
         log(f"serving {name}: capturing graphs")
@@ -447,6 +450,16 @@ def serve_bench(args):
         torch.cuda.empty_cache()
     if args.serve_projections:
         out["with_projection_requests"] = serve_mixed_bench(args, geoms, clean)
+    return out
+
+
+def _chunked(args):
+    """Request classes reconstructed in view ranges (uploads overlapping the reconstruction)."""
+    out = {}
+    if args.serve_chunks > 1:
+        out["cone512"] = args.serve_chunks
+    if args.serve_chunks_256 > 1:
+        out["cone256"] = args.serve_chunks_256
     return out
 
 
@@ -498,8 +511,7 @@ def serve_mixed_bench(args, geoms, clean):
                     for k, s_ in clean.items()}
         payloads["cone256_proj"] = [vol.to(storage).cpu().pin_memory()]
         payloads["fan512_proj"] = [img.to(storage).cpu().pin_memory()]
-        ex = GpuExecutor(plans, slots=args.serve_slots, use_graphs=True,
-                         chunked={"cone512": args.serve_chunks} if args.serve_chunks > 1 else {})
+        ex = GpuExecutor(plans, slots=args.serve_slots, use_graphs=True, chunked=_chunked(args))
         ex.warm()
         reqs = poisson_arrivals(9, args.serve_rate, args.serve_requests, mix)
         for r in reqs:
