@@ -734,7 +734,7 @@ __device__ __forceinline__ void fwd_block(const FwdProblem &P, long long q) {
 // the views so that a launch stays within the dispatch grid's 2^32 work-items); P.sino,
 // P.inv_ar and P.src point at the range's first view of item 0.  cell_stride is the
 // elements per batch item of P.cells (the volume's with DIRECT).
-template <typename TS, bool HW, typename IT, bool DIRECT = false>
+template <typename TS, bool HW, typename IT, bool DIRECT = false, typename TL = FwdTile16>
 __global__ void __launch_bounds__(256)
 cone_fwd_kernel(FwdProblem P, long long cell_stride, int n_all) {
   // blockIdx.z selects the batch item; items share shapes and have their own geometry
@@ -745,7 +745,7 @@ cone_fwd_kernel(FwdProblem P, long long cell_stride, int n_all) {
   P.src += (size_t)b * n_all * 3;
   P.step += b;
   P.spacing += (size_t)b * 3;
-  fwd_block<TS, HW, IT, DIRECT>(P, blockIdx.x);
+  fwd_block<TS, HW, IT, DIRECT, TL>(P, blockIdx.x);
 }
 
 // Up to this many problems of different shapes run in one persistent launch.
@@ -1117,9 +1117,20 @@ static long long fwd_blocks(int n, int Hd, int Wd, int tile = 0) {
   return (long long)kXcds * ((tiles_y + kXcds - 1) / kXcds) * n * tiles_x;
 }
 // Views per forward launch (0: one view alone exceeds a launch).
-static int fwd_views_per_launch(int n, int Hd, int Wd) {
-  const long long per = kMaxGridX / fwd_blocks(1, Hd, Wd);
+static int fwd_views_per_launch(int n, int Hd, int Wd, int tile = 0) {
+  const long long per = kMaxGridX / fwd_blocks(1, Hd, Wd, tile);
   return (int)(per < n ? per : n);
+}
+
+// Tile shape (kFwdTiles) of the single-problem forward with a cell copy: 32 x 8 pixels of
+// 8 x 8 waves for f32 cells, and for 16-bit cells of volumes up to 2^25 voxels; 16 x 16 of
+// 4 x 16 waves for larger 16-bit volumes.  Measured per problem (phantom volumes, 16 x 16 /
+// 4 x 16 against 32 x 8 / 8 x 8): f32 equal on the 512^3 / 720-view / 1024 x 768 workload
+// (372.7 / 372.9 ms) and 3-11% faster on the others (256^3 / 360 views / 400^2: 27.8 / 24.8
+// ms); bf16 faster up to 320^3 (43.8 / 41.0 ms), equal at 384^3 and slower beyond (512^3:
+// 213.2 / 237.6 ms).  The shape changes which lane computes which ray, never a result.
+template <typename TS> static int fwd_single_tile(int D, int H, int W) {
+  return (sizeof(TS) == 4 || (long long)D * H * W <= (1LL << 25)) ? 2 : 0;
 }
 
 template <typename TS>
@@ -1132,11 +1143,19 @@ static void launch_cells(const void *vol, void *cells, int B, int D, int H, int 
 }
 
 template <typename TS, typename IT>
-static void launch_fwd_it(bool hw, const FwdProblem &P, dim3 grid, long long cs, int n_all, hipStream_t stream) {
-  if (hw)
+static void launch_fwd_it(bool hw, int tile, const FwdProblem &P, dim3 grid, long long cs, int n_all,
+                          hipStream_t stream) {
+  using T2 = FwdTile<32, 8, 8>;
+  if (tile == 2) {
+    if (hw)
+      cone_fwd_kernel<TS, true, IT, false, T2><<<grid, dim3(256), 0, stream>>>(P, cs, n_all);
+    else
+      cone_fwd_kernel<TS, false, IT, false, T2><<<grid, dim3(256), 0, stream>>>(P, cs, n_all);
+  } else if (hw) {
     cone_fwd_kernel<TS, true, IT><<<grid, dim3(256), 0, stream>>>(P, cs, n_all);
-  else
+  } else {
     cone_fwd_kernel<TS, false, IT><<<grid, dim3(256), 0, stream>>>(P, cs, n_all);
+  }
 }
 
 template <typename TS>
@@ -1145,22 +1164,24 @@ static void launch_fwd(bool hw, const void *cells, float *sino, const float *inv
                        int H, int W, int n, int Hd, int Wd, hipStream_t stream) {
   const long long cs = (long long)cell_elems<TS>(D, H, W);
   const bool small = fwd_small_index(D, H, W);
-  const int per = fwd_views_per_launch(n, Hd, Wd);
+  const int tile = fwd_single_tile<TS>(D, H, W);
+  const int tc = fwd_tile_cols(tile), tr = fwd_tile_rows(tile);
+  const int per = fwd_views_per_launch(n, Hd, Wd, tile);
   for (int v0 = 0; v0 < n; v0 += per) {
     const int k = n - v0 < per ? n - v0 : per;
     FwdProblem P{cells, sino + (size_t)v0 * Hd * Wd, inv_ar + (size_t)v0 * 9, src + (size_t)v0 * 3, step,
-                 spacing, W, H, D, k, Hd, Wd, (Wd + 15) / 16, (Hd + 15) / 16, 0};
-    dim3 grid((unsigned)fwd_blocks(k, Hd, Wd), 1, B);
+                 spacing, W, H, D, k, Hd, Wd, (Wd + tc - 1) / tc, (Hd + tr - 1) / tr, 0};
+    dim3 grid((unsigned)fwd_blocks(k, Hd, Wd, tile), 1, B);
     if constexpr (sizeof(TS) == 2) {
       if (fwd_byte_offsets(D, H, W)) {
-        launch_fwd_it<TS, unsigned>(hw, P, grid, cs, n, stream);
+        launch_fwd_it<TS, unsigned>(hw, tile, P, grid, cs, n, stream);
         continue;
       }
     }
     if (small)
-      launch_fwd_it<TS, int>(hw, P, grid, cs, n, stream);
+      launch_fwd_it<TS, int>(hw, tile, P, grid, cs, n, stream);
     else
-      launch_fwd_it<TS, long long>(hw, P, grid, cs, n, stream);
+      launch_fwd_it<TS, long long>(hw, tile, P, grid, cs, n, stream);
   }
 }
 
@@ -1287,7 +1308,7 @@ static int fwd_check_args(const char *name, const void *volume, int volume_dtype
     set_error("%s: volume %dx%dx%d: a side of 2^24 - 3 voxels or more", name, D, H, W);
     return PYRONN_ERR_TOO_LARGE;
   }
-  if (fwd_views_per_launch(n_views, det_h, det_w) < 1) {
+  if (fwd_views_per_launch(n_views, det_h, det_w, 0) < 1 || fwd_views_per_launch(n_views, det_h, det_w, 2) < 1) {
     set_error("%s: detector %dx%d exceeds one launch", name, det_h, det_w);
     return PYRONN_ERR_TOO_LARGE;
   }
