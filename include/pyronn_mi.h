@@ -256,7 +256,7 @@ int pyronn_cone_projection3d_batched(const pyronn_cone_problem *problems, int co
 
 size_t pyronn_cone_backprojection3d_batched_workspace_size(const pyronn_cone_problem *problems,
                                                            int count, int dtype);
-/* z_per_lane: voxels along z per lane, 4 or 8 (<= 0: 8 for f32, 4 for 16-bit storage).
+/* z_per_lane: voxels along z per lane, 4 or 8 (<= 0: 8 for f32 and bf16, 4 for f16 storage).
  * blocks_per_cu > 0: a persistent grid of that many workgroups per CU claiming items from
  * per-XCD counters; <= 0 (default): one workgroup per item in item order.  Both give the
  * same bits; one workgroup per item keeps the single kernel's register budget and was

@@ -1205,9 +1205,10 @@ static void launch_fwd_direct(bool hw, const void *vol, float *sino, const float
   }
 }
 
-// z voxels per lane: 8 for f32 detector cells, 4 for the 16-bit ones (measured
-// on the 512^3 / 720-view / 1024x768 workload).
-template <typename TS> constexpr int bp_zpt() { return sizeof(TS) == 4 ? 8 : 4; }
+// z voxels per lane: 8 for f32 and bf16 detector cells, 4 for f16 ones (measured on the
+// 512^3 / 720-view / 1024x768 workload: f32 63.0 against 70.9 ms with 4, bf16 53.7 against
+// 57.7 ms, f16 56.9 ms with 4 against 125.9 ms with 8).
+template <typename TS> constexpr int bp_zpt() { return std::is_same<TS, _Float16>::value ? 4 : 8; }
 
 template <typename TS>
 static void launch_det_cells(const void *sino, void *cells, int images, int Hd, int Wd,
@@ -1802,7 +1803,7 @@ extern "C" int pyronn_cone_backprojection3d_batched(const pyronn_cone_problem *p
   PYRONN_CHECK_ARG(((uintptr_t)workspace & 15) == 0, "cone_backprojection3d_batched: workspace must be 16-byte aligned");
   hipStream_t s = (hipStream_t)stream;
   const bool hw = hardware_interp != 0;
-  const int z = z_per_lane > 0 ? z_per_lane : (dtype == PYRONN_F32 ? 8 : 4);
+  const int z = z_per_lane > 0 ? z_per_lane : (dtype == PYRONN_F16 ? 4 : 8);
   char *ws = (char *)workspace;
   if (dtype == PYRONN_F32)
     rc = z == 8 ? bp_batched<float, 8>(problems, count, hw, blocks_per_cu, ws, s)
