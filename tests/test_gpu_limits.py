@@ -161,6 +161,39 @@ def test_forward_views_beyond_one_dispatch():
     torch.cuda.empty_cache()
 
 
+def test_batched_forward_items_beyond_one_dispatch():
+    """Two problems of 2048 x 2048 pixels, 600 and 500 views, in one batched call with one
+    workgroup per item: 16384 items per view, 1.8e7 in all, more than one dispatch holds, so
+    the work list runs as two launches.  The split falls in the second problem's
+    fourteenth band of tile rows (a problem's items run band by band, each band over its tile
+    columns and, inside each, over the views), so every view of that problem has tiles in
+    both launches; whole views of both problems are checked."""
+    from pyronn_layers import batched
+    specs = [((32, 32, 32), 600, 7), ((20, 28, 24), 500, 8)]
+    geoms, vols = [], []
+    for shape, n, seed in specs:
+        geoms.append(geo.ConeGeometry(shape, (1.0, 1.0, 1.0), (2048, 2048), (0.05, 0.05), n, 100.0, 200.0,
+                                      step_size=0.5))
+        vols.append(_rand_volume(shape, seed))
+    items = sum(8 * ((2048 // 16 + 7) // 8) * (2048 // 16) * g.n_views for g in geoms)
+    assert items > (1 << 24) - 1
+    print(f"batched forward, {items} items", flush=True)
+    sinos = batched.cone_projection3d_batched(vols, geoms, False, blocks_per_cu=0, forward_tile=0)
+    torch.cuda.synchronize()
+    for k, (g, v) in enumerate(zip(geoms, vols)):
+        P = g.projection_matrices().astype(np.float32)
+        sp = np.asarray(g.volume_spacing, np.float32)
+        og = np.asarray(g.volume_origin, np.float32)
+        inv, src, _, _, _ = orc.cone_setup(P, sp, og)
+        vh = v.cpu().numpy()
+        for view in (0, g.n_views // 2, g.n_views - 1):
+            ref, _, ns = orc.cone_forward(vh, inv, src, g.detector_shape, sp, 0.5, views=(view, view + 1))
+            assert ns > 0
+            assert np.array_equal(sinos[k][view:view + 1].cpu().numpy(), ref), f"problem {k} view {view}"
+    del sinos
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("batch", [1, 2])
 def test_backprojection_z_blocks_beyond_one_launch(batch):
     """600000 (batch 1) or 300000 (batch 2) slices: more than 65535 z blocks x items, two
