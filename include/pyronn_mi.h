@@ -227,9 +227,11 @@ int pyronn_fan_backprojection2d(const void *sinogram, int sinogram_dtype, int ba
  * No single counterpart in the reference: ConeProjection3DOp::Compute
  * (cc/ops/cone_projector_3D_OPKernel.cc:234-247) loops over batch items of one
  * shape and launches once per item.  Here up to PYRONN_MAX_BATCHED_PROBLEMS
- * volumes / sinograms of any shapes run in ONE persistent launch (plus small
- * per-problem layout passes); every output equals the single-problem call bit
- * for bit.  All problems share the storage dtype and hardware_interp.
+ * volumes / sinograms of any shapes run in ONE launch (plus small per-problem
+ * layout passes; a forward of more than 2^24 - 1 workgroups, one per item, takes
+ * several launches in item order, and such a backprojection the persistent grid);
+ * every output equals the single-problem call bit for bit.  All problems share
+ * the storage dtype and hardware_interp.
  * -------------------------------------------------------------------------- */
 typedef struct pyronn_cone_problem {
   const void *input;                  /* forward: volume [D][H][W]; backprojection: sinogram [n][H_d][W_d] */
@@ -248,8 +250,11 @@ size_t pyronn_cone_projection3d_batched_workspace_size(const pyronn_cone_problem
  * blocks_per_cu > 0: a persistent grid of that many workgroups per CU claiming items from
  *   per-XCD counters; <= 0 (default): one workgroup per item in item order (no claims);
  * forward_tile: the detector tile of a 256-lane workgroup and the waves within it,
- *   0: 16 x 16 pixels, waves of 4 columns x 16 rows (the single-problem shape),
- *   1: 8 x 32, waves 2 x 32;  2: 32 x 8, waves 8 x 8;  3: 16 x 16, waves 8 x 8. */
+ *   0: 16 x 16 pixels, waves of 4 columns x 16 rows (the single-problem shape of
+ *      16-bit volumes above 2^25 voxels),
+ *   1: 8 x 32, waves 2 x 32;
+ *   2: 32 x 8, waves 8 x 8 (the single-problem shape of f32 and smaller 16-bit volumes);
+ *   3: 16 x 16, waves 8 x 8. */
 int pyronn_cone_projection3d_batched(const pyronn_cone_problem *problems, int count, int dtype,
                                      int hardware_interp, int blocks_per_cu, int forward_tile,
                                      void *workspace, size_t workspace_bytes, pyronn_stream_t stream);
@@ -258,7 +263,8 @@ size_t pyronn_cone_backprojection3d_batched_workspace_size(const pyronn_cone_pro
                                                            int count, int dtype);
 /* z_per_lane: voxels along z per lane, 4 or 8 (<= 0: 8 for f32 and bf16, 4 for f16 storage).
  * blocks_per_cu > 0: a persistent grid of that many workgroups per CU claiming items from
- * per-XCD counters; <= 0 (default): one workgroup per item in item order.  Both give the
+ * per-XCD counters; <= 0 (default): one workgroup per item in item order (the persistent
+ * grid of 8 per CU when the items exceed one dispatch's 2^24 - 1 workgroups).  Both give the
  * same bits; one workgroup per item keeps the single kernel's register budget and was
  * measured faster (200.6 against 217.2 ms on an eight-problem f32 batch). */
 int pyronn_cone_backprojection3d_batched(const pyronn_cone_problem *problems, int count, int dtype,
